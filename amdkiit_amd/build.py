@@ -1,0 +1,63 @@
+"""Build libamdk.so (HIP, gfx950) and the Fortran binding module in-tree.
+
+Usage: python -m amdkiit_amd.build [--check]
+The library is written next to this file (amdkiit_amd/libamdk.so) so that it
+travels with the source tree; nothing is installed.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "libamdk.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FC = os.environ.get("AMDK_FC", "/opt/rocm/bin/amdflang")
+
+SOURCES = ["amdk_api.hip"]
+
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def build_lib(verbose=True):
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs + [
+        "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_fortran(verbose=True):
+    """Compile the ISO_C_BINDING module and its smoke driver (host code only)."""
+    fdir = os.path.join(HERE, "fortran")
+    srcs = [os.path.join(fdir, "amdk_hip.f90"), os.path.join(fdir, "amdk_smoke.f90")]
+    if not all(os.path.exists(s) for s in srcs) or not shutil.which(FC):
+        return None
+    out = os.path.join(fdir, "amdk_smoke")
+    moddir = os.path.join(fdir, "_mod")
+    os.makedirs(moddir, exist_ok=True)
+    cmd = [FC, "-O2", "-J", moddir, "-o", out] + srcs + [
+        "-L" + HERE, "-lamdk", "-Wl,-rpath," + HERE]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return out
+
+
+def main():
+    build_lib()
+    build_fortran()
+
+
+if __name__ == "__main__":
+    sys.exit(main())

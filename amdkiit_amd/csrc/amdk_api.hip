@@ -1,0 +1,619 @@
+// libamdk: context creation, plane-wave maps, transfers and the H|psi> /
+// density drivers behind the C ABI declared in include/amdk.h.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <tuple>
+
+#include "amdk_ctx.h"
+
+using namespace amdk;
+
+namespace amdk {
+
+FftPlan1D make_plan(int n) {
+  FftPlan1D p;
+  p.n = n;
+  p.nst = 0;
+  if (n < 2) throw Error(AMDK_ERR_ARG, "FFT length must be >= 2");
+  int m = n;
+  while (m % 4 == 0 && m > 4) { p.rad[p.nst++] = 4; m /= 4; }
+  const int rs[4] = {5, 3, 4, 2};
+  for (int r : rs) {
+    while (m % r == 0 && m > 1) {
+      if (p.nst >= AMDK_MAX_STAGES) throw Error(AMDK_ERR_ARG, "FFT length has too many factors");
+      p.rad[p.nst++] = r;
+      m /= r;
+    }
+  }
+  if (m != 1) throw Error(AMDK_ERR_ARG, "FFT length " + std::to_string(n) + " is not 2^a 3^b 5^c");
+  return p;
+}
+
+void* dev_alloc(amdk_ctx* c, size_t bytes) {
+  void* p = nullptr;
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) throw Error(AMDK_ERR_OOM, "hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
+  c->owned.push_back(p);
+  AMDK_TRY_HIP(hipMemsetAsync(p, 0, bytes, c->stream));
+  return p;
+}
+
+size_t cbytes(const amdk_ctx* c) { return c->prec == AMDK_PREC_FP64 ? sizeof(double2) : sizeof(float2); }
+
+int prof_begin(amdk_ctx* c, const char* name) {
+  if (!c->prof) return -1;
+  ProfSlot s;
+  s.name = name;
+  for (hipEvent_t* e : {&s.a, &s.b}) {
+    if (!c->event_pool.empty()) {
+      *e = c->event_pool.back();
+      c->event_pool.pop_back();
+    } else {
+      AMDK_TRY_HIP(hipEventCreate(e));
+    }
+  }
+  AMDK_TRY_HIP(hipEventRecord(s.a, c->stream));
+  c->pending.push_back(s);
+  return (int)c->pending.size() - 1;
+}
+
+void prof_end(amdk_ctx* c, int slot) {
+  if (slot < 0) return;
+  AMDK_TRY_HIP(hipEventRecord(c->pending[slot].b, c->stream));
+}
+
+void sync(amdk_ctx* c) {
+  AMDK_TRY_HIP(hipStreamSynchronize(c->stream));
+  for (auto& s : c->pending) {
+    float ms = 0;
+    AMDK_TRY_HIP(hipEventElapsedTime(&ms, s.a, s.b));
+    auto& acc = c->prof_acc[s.name];
+    acc.first += ms;
+    acc.second += 1;
+    c->event_pool.push_back(s.a);
+    c->event_pool.push_back(s.b);
+  }
+  c->pending.clear();
+}
+
+}  // namespace amdk
+
+// ---------------------------------------------------------------------------
+// Sphere construction (host).  Canonicalise every caller G to the z-half,
+// sort by (m3, m2, m1) and cut into half rows.
+static void build_sphere(amdk_ctx* c, const int* mill, int ncaller, Sphere& S) {
+  if (ncaller <= 0 || !mill) throw Error(AMDK_ERR_ARG, "empty Miller list");
+  std::vector<std::array<int, 3>> m(ncaller);
+  S.h_conj_caller.assign(ncaller, 0);
+  for (int i = 0; i < ncaller; ++i) {
+    int a = mill[3 * i], b = mill[3 * i + 1], d = mill[3 * i + 2];
+    const bool zhalf = d > 0 || (d == 0 && (b > 0 || (b == 0 && a >= 0)));
+    if (!zhalf) { a = -a; b = -b; d = -d; S.h_conj_caller[i] = 1; }
+    if (2 * std::abs(a) >= c->n1 || 2 * std::abs(b) >= c->n2 || 2 * std::abs(d) >= c->n3)
+      throw Error(AMDK_ERR_ARG, "Miller index outside the FFT grid (|m| must be < n/2)");
+    m[i] = {a, b, d};
+  }
+  std::vector<int> ord(ncaller);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::sort(ord.begin(), ord.end(), [&](int x, int y) {
+    return std::make_tuple(m[x][2], m[x][1], m[x][0]) < std::make_tuple(m[y][2], m[y][1], m[y][0]);
+  });
+  S.npw = ncaller;
+  S.ncaller = ncaller;
+  S.ldc = (ncaller + 63) / 64 * 64;
+  S.h_from_caller.assign(ncaller, 0);
+  S.h_mill.assign(3 * ncaller, 0);
+  for (int k = 0; k < ncaller; ++k) {
+    const int i = ord[k];
+    if (k > 0 && m[ord[k - 1]] == m[i]) throw Error(AMDK_ERR_ARG, "duplicate plane wave (G or -G listed twice)");
+    S.h_from_caller[i] = k;
+    for (int t = 0; t < 3; ++t) S.h_mill[3 * k + t] = m[i][t];
+  }
+  // half rows
+  S.h_rows.clear();
+  for (int k = 0; k < ncaller;) {
+    RowInfo r{};
+    r.start = k;
+    r.m1_0 = S.h_mill[3 * k];
+    r.m2 = S.h_mill[3 * k + 1];
+    r.m3 = S.h_mill[3 * k + 2];
+    int j = k + 1;
+    while (j < ncaller && S.h_mill[3 * j + 1] == r.m2 && S.h_mill[3 * j + 2] == r.m3) {
+      if (S.h_mill[3 * j] != S.h_mill[3 * (j - 1)] + 1)
+        throw Error(AMDK_ERR_ARG, "plane-wave set is not convex along m1 (row has a gap)");
+      ++j;
+    }
+    r.cnt = j - k;
+    S.h_rows.push_back(r);
+    k = j;
+  }
+  S.nrow = (int)S.h_rows.size();
+  // full rows -> slots sorted by (m3, m2)
+  struct FR { int m3, m2, row, mirror; };
+  std::vector<FR> fr;
+  S.R = 0;
+  for (int r = 0; r < S.nrow; ++r) {
+    const RowInfo& ri = S.h_rows[r];
+    fr.push_back({ri.m3, ri.m2, r, 0});
+    if (!(ri.m2 == 0 && ri.m3 == 0)) fr.push_back({-ri.m3, -ri.m2, r, 1});
+    S.R = std::max(S.R, std::abs(ri.m3));
+  }
+  std::sort(fr.begin(), fr.end(), [](const FR& x, const FR& y) { return std::make_pair(x.m3, x.m2) < std::make_pair(y.m3, y.m2); });
+  S.nslot = (int)fr.size();
+  S.nplane = 2 * S.R + 1;
+  S.h_slot_m2.assign(S.nslot, 0);
+  S.h_plane_off.assign(S.nplane + 1, 0);
+  for (int s = 0; s < S.nslot; ++s) {
+    S.h_slot_m2[s] = fr[s].m2;
+    RowInfo& ri = S.h_rows[fr[s].row];
+    if (fr[s].mirror) ri.slot_m = s; else ri.slot_d = s;
+    S.h_plane_off[fr[s].m3 + S.R + 1] += 1;
+  }
+  for (int r = 0; r < S.nrow; ++r)
+    if (S.h_rows[r].m2 == 0 && S.h_rows[r].m3 == 0) S.h_rows[r].slot_m = -1;
+  for (int p = 0; p < S.nplane; ++p) S.h_plane_off[p + 1] += S.h_plane_off[p];
+  if (S.h_plane_off[S.nplane] != S.nslot) throw Error(AMDK_ERR_ARG, "internal: slot count mismatch");
+  // upload
+  S.rows = (RowInfo*)dev_alloc(c, sizeof(RowInfo) * S.nrow);
+  S.slot_m2 = (int*)dev_alloc(c, sizeof(int) * S.nslot);
+  S.plane_off = (int*)dev_alloc(c, sizeof(int) * (S.nplane + 1));
+  S.d_from_caller = (int*)dev_alloc(c, sizeof(int) * ncaller);
+  S.d_conj_caller = (int*)dev_alloc(c, sizeof(int) * ncaller);
+  AMDK_TRY_HIP(hipMemcpy(S.rows, S.h_rows.data(), sizeof(RowInfo) * S.nrow, hipMemcpyHostToDevice));
+  AMDK_TRY_HIP(hipMemcpy(S.slot_m2, S.h_slot_m2.data(), sizeof(int) * S.nslot, hipMemcpyHostToDevice));
+  AMDK_TRY_HIP(hipMemcpy(S.plane_off, S.h_plane_off.data(), sizeof(int) * (S.nplane + 1), hipMemcpyHostToDevice));
+  AMDK_TRY_HIP(hipMemcpy(S.d_from_caller, S.h_from_caller.data(), sizeof(int) * ncaller, hipMemcpyHostToDevice));
+  AMDK_TRY_HIP(hipMemcpy(S.d_conj_caller, S.h_conj_caller.data(), sizeof(int) * ncaller, hipMemcpyHostToDevice));
+}
+
+static PassGeom geom_of(const amdk_ctx* c, const Sphere& S) {
+  PassGeom g;
+  g.n1 = c->n1; g.n2 = c->n2; g.n3 = c->n3;
+  g.nrow = S.nrow; g.nslot = S.nslot; g.nplane = S.nplane; g.R = S.R;
+  g.rows = S.rows; g.slot_m2 = S.slot_m2; g.plane_off = S.plane_off;
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// transfer kernels between caller-order FP64 arrays and internal arrays
+template <typename V>
+__global__ void k_from_caller(const double2* __restrict__ src, int ldsrc, V* __restrict__ dst, int ldc,
+                              const int* __restrict__ pos, const int* __restrict__ cj, int n, int nb) {
+  const int b = blockIdx.y;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double2 v = src[(size_t)b * ldsrc + i];
+    if (cj[i]) v.y = -v.y;
+    V o;
+    o.x = v.x;
+    o.y = v.y;
+    dst[(size_t)b * ldc + pos[i]] = o;
+  }
+}
+template <typename V>
+__global__ void k_to_caller(const V* __restrict__ src, int ldc, double2* __restrict__ dst, int lddst,
+                            const int* __restrict__ pos, const int* __restrict__ cj, int n, int nb) {
+  const int b = blockIdx.y;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const V v = src[(size_t)b * ldc + pos[i]];
+    double2 o;
+    o.x = v.x;
+    o.y = cj[i] ? -(double)v.y : (double)v.y;
+    dst[(size_t)b * lddst + i] = o;
+  }
+}
+
+template <typename T>
+__global__ void k_scale_copy(const double* __restrict__ src, T* __restrict__ dst, size_t n, double s) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = (T)(src[i] * s);
+}
+
+static void bands_in(amdk_ctx* c, const double* host, int ldsrc, int nb, void* dst) {
+  const Sphere& S = c->spw;
+  const size_t nsrc = (size_t)ldsrc * nb * 2;
+  double* stage = nullptr;
+  AMDK_TRY_HIP(hipMalloc(&stage, sizeof(double) * std::max<size_t>(nsrc, 2)));
+  AMDK_TRY_HIP(hipMemcpyAsync(stage, host, sizeof(double) * nsrc, hipMemcpyHostToDevice, c->stream));
+  AMDK_TRY_HIP(hipMemsetAsync(dst, 0, cbytes(c) * (size_t)S.ldc * nb, c->stream));
+  dim3 grid((S.npw + 255) / 256, nb);
+  if (c->prec == AMDK_PREC_FP64)
+    k_from_caller<double2><<<grid, 256, 0, c->stream>>>((const double2*)stage, ldsrc, (double2*)dst, S.ldc, S.d_from_caller, S.d_conj_caller, S.npw, nb);
+  else
+    k_from_caller<float2><<<grid, 256, 0, c->stream>>>((const double2*)stage, ldsrc, (float2*)dst, S.ldc, S.d_from_caller, S.d_conj_caller, S.npw, nb);
+  AMDK_TRY_HIP(hipGetLastError());
+  AMDK_TRY_HIP(hipStreamSynchronize(c->stream));
+  AMDK_TRY_HIP(hipFree(stage));
+}
+
+static void bands_out(amdk_ctx* c, const void* src, int nb, double* host, int lddst) {
+  const Sphere& S = c->spw;
+  const size_t ndst = (size_t)lddst * nb * 2;
+  double* stage = nullptr;
+  AMDK_TRY_HIP(hipMalloc(&stage, sizeof(double) * std::max<size_t>(ndst, 2)));
+  AMDK_TRY_HIP(hipMemsetAsync(stage, 0, sizeof(double) * ndst, c->stream));
+  dim3 grid((S.npw + 255) / 256, nb);
+  if (c->prec == AMDK_PREC_FP64)
+    k_to_caller<double2><<<grid, 256, 0, c->stream>>>((const double2*)src, S.ldc, (double2*)stage, lddst, S.d_from_caller, S.d_conj_caller, S.npw, nb);
+  else
+    k_to_caller<float2><<<grid, 256, 0, c->stream>>>((const float2*)src, S.ldc, (double2*)stage, lddst, S.d_from_caller, S.d_conj_caller, S.npw, nb);
+  AMDK_TRY_HIP(hipGetLastError());
+  AMDK_TRY_HIP(hipMemcpyAsync(host, stage, sizeof(double) * ndst, hipMemcpyDeviceToHost, c->stream));
+  AMDK_TRY_HIP(hipStreamSynchronize(c->stream));
+  AMDK_TRY_HIP(hipFree(stage));
+}
+
+// ---------------------------------------------------------------------------
+// tile shapes per precision
+template <typename T> struct Tiles;
+template <> struct Tiles<float> { static constexpr int RPB = 8, TX = 16, PG = 4; };
+template <> struct Tiles<double> { static constexpr int RPB = 4, TX = 8, PG = 2; };
+
+template <typename T>
+static void launch_rows_inv(amdk_ctx* c, const Sphere& S, const void* C, int ldc, int nb, int pair0, int np) {
+  using V = typename cplx_of<T>::type;
+  constexpr int RPB = Tiles<T>::RPB;
+  const size_t lds = (size_t)(c->n1 + 4 * RPB * c->n1) * sizeof(V);
+  dim3 grid((S.nrow + RPB - 1) / RPB, np);
+  const int ps = prof_begin(c, "rows_inv");
+  k_rows_inv<T, RPB><<<grid, 256, lds, c->stream>>>(geom_of(c, S), (const V*)C, ldc, nb, pair0, (V*)c->W1, c->p1,
+                                                     (const V*)(sizeof(T) == 4 ? c->tw_f[0] : c->tw_d[0]));
+  AMDK_TRY_HIP(hipGetLastError());
+  prof_end(c, ps);
+}
+
+template <typename T>
+static void launch_cols(amdk_ctx* c, const Sphere& S, int np, bool inverse) {
+  using V = typename cplx_of<T>::type;
+  constexpr int TX = Tiles<T>::TX;
+  const size_t lds = (size_t)(c->n2 + 2 * c->n2 * TX) * sizeof(V);
+  dim3 grid((c->n1 + TX - 1) / TX, S.nplane, np);
+  const V* tw = (const V*)(sizeof(T) == 4 ? c->tw_f[1] : c->tw_d[1]);
+  if (inverse) {
+    const int ps = prof_begin(c, "cols_inv");
+    k_cols_inv<T, TX><<<grid, 256, lds, c->stream>>>(geom_of(c, S), (const V*)c->W1, (V*)c->W2, c->p2, tw);
+    AMDK_TRY_HIP(hipGetLastError());
+    prof_end(c, ps);
+  } else {
+    const int ps = prof_begin(c, "cols_fwd");
+    k_cols_fwd<T, TX><<<grid, 256, lds, c->stream>>>(geom_of(c, S), (const V*)c->W2, (V*)c->W1, c->p2, tw);
+    AMDK_TRY_HIP(hipGetLastError());
+    prof_end(c, ps);
+  }
+}
+
+template <typename T>
+static void launch_z_vpsi(amdk_ctx* c, const Sphere& S, int np) {
+  using V = typename cplx_of<T>::type;
+  constexpr int TX = Tiles<T>::TX, PG = Tiles<T>::PG;
+  const size_t lds = (size_t)(c->n3 + 2 * c->n3 * TX) * sizeof(V) + (size_t)c->n3 * TX * sizeof(T);
+  dim3 grid((c->n1 + TX - 1) / TX, c->n2, (np + PG - 1) / PG);
+  const int ps = prof_begin(c, "z_vpsi");
+  k_z_vpsi<T, TX><<<grid, 256, lds, c->stream>>>(geom_of(c, S), (V*)c->W2, (const T*)c->vr, np, PG, c->p3,
+                                                 (const V*)(sizeof(T) == 4 ? c->tw_f[2] : c->tw_d[2]));
+  AMDK_TRY_HIP(hipGetLastError());
+  prof_end(c, ps);
+}
+
+template <typename T>
+static void launch_rows_fwd_hpsi(amdk_ctx* c, const Sphere& S, const void* C, void* C2, int nb, int pair0, int np) {
+  using V = typename cplx_of<T>::type;
+  constexpr int RPB = Tiles<T>::RPB;
+  const size_t lds = (size_t)(c->n1 + 4 * RPB * c->n1) * sizeof(V);
+  dim3 grid((S.nrow + RPB - 1) / RPB, np);
+  const int ps = prof_begin(c, "rows_fwd_hpsi");
+  k_rows_fwd_hpsi<T, RPB><<<grid, 256, lds, c->stream>>>(geom_of(c, S), (const V*)c->W1, (const V*)C, (V*)C2, S.ldc, nb,
+                                                          pair0, c->d_occ, c->rc, c->p1,
+                                                          (const V*)(sizeof(T) == 4 ? c->tw_f[0] : c->tw_d[0]));
+  AMDK_TRY_HIP(hipGetLastError());
+  prof_end(c, ps);
+}
+
+template <typename T>
+static void launch_z_rho(amdk_ctx* c, const Sphere& S, int np, double* rho, int accumulate) {
+  using V = typename cplx_of<T>::type;
+  constexpr int TX = Tiles<T>::TX;
+  const size_t lds = (size_t)(c->n3 + 2 * c->n3 * TX) * sizeof(V);
+  dim3 grid((c->n1 + TX - 1) / TX, c->n2);
+  const int ps = prof_begin(c, "z_rho");
+  k_z_rho<T, TX, 16><<<grid, 256, lds, c->stream>>>(geom_of(c, S), (const V*)c->W2, rho, np, c->d_wpair, accumulate, c->p3,
+                                                     (const V*)(sizeof(T) == 4 ? c->tw_f[2] : c->tw_d[2]));
+  AMDK_TRY_HIP(hipGetLastError());
+  prof_end(c, ps);
+}
+
+template <typename T>
+static void run_vpsi(amdk_ctx* c, const void* C0, void* C2, int nb) {
+  const Sphere& S = c->spw;
+  const int npairs = (nb + 1) / 2;
+  for (int p0 = 0; p0 < npairs; p0 += c->cap_pairs) {
+    const int np = std::min(c->cap_pairs, npairs - p0);
+    launch_rows_inv<T>(c, S, C0, S.ldc, nb, p0, np);
+    launch_cols<T>(c, S, np, true);
+    launch_z_vpsi<T>(c, S, np);
+    launch_cols<T>(c, S, np, false);
+    launch_rows_fwd_hpsi<T>(c, S, C0, C2, nb, p0, np);
+  }
+}
+
+template <typename T>
+static void run_density(amdk_ctx* c, const void* C0, const double* occ, int nb, double* rho) {
+  const Sphere& S = c->spw;
+  const int npairs = (nb + 1) / 2;
+  std::vector<double> w(2 * npairs, 0.0);
+  for (int b = 0; b < nb; ++b) w[b] = occ[b] / c->omega;
+  for (int p0 = 0; p0 < npairs; p0 += c->cap_pairs) {
+    const int np = std::min(c->cap_pairs, npairs - p0);
+    AMDK_TRY_HIP(hipMemcpyAsync(c->d_wpair, w.data() + 2 * p0, sizeof(double) * 2 * np, hipMemcpyHostToDevice, c->stream));
+    launch_rows_inv<T>(c, S, C0, S.ldc, nb, p0, np);
+    launch_cols<T>(c, S, np, true);
+    launch_z_rho<T>(c, S, np, rho, p0 > 0);
+    AMDK_TRY_HIP(hipStreamSynchronize(c->stream));  // d_wpair reused by the next chunk
+  }
+}
+
+// ---------------------------------------------------------------------------
+#define API_BEGIN \
+  if (!ctx) return AMDK_ERR_ARG; \
+  try {
+#define API_END                                  \
+  }                                              \
+  catch (const amdk::Error& e) {                 \
+    ctx->err = e.what();                         \
+    return e.code;                               \
+  }                                              \
+  catch (const std::exception& e) {              \
+    ctx->err = e.what();                         \
+    return AMDK_ERR_DEVICE;                      \
+  }                                              \
+  return AMDK_OK;
+
+static void check_nb(amdk_ctx* c, int nb) {
+  if (nb < 1 || nb > c->nbmax) throw Error(AMDK_ERR_ARG, "band count outside 1..nbmax");
+}
+
+static void check_finite(const double* p, size_t n, const char* what) {
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(p[i])) throw Error(AMDK_ERR_NONFINITE, std::string("non-finite value in ") + what);
+}
+
+extern "C" {
+
+int amdk_ctx_create(const amdk_desc* d, amdk_ctx** out) {
+  if (!out) return AMDK_ERR_ARG;
+  *out = nullptr;
+  amdk_ctx* ctx = new amdk_ctx();
+  int rc = AMDK_OK;
+  try {
+    if (!d) throw Error(AMDK_ERR_ARG, "null descriptor");
+    if (d->precision != AMDK_PREC_FP64 && d->precision != AMDK_PREC_MIXED) throw Error(AMDK_ERR_ARG, "bad precision mode");
+    if (d->nbmax < 1) throw Error(AMDK_ERR_ARG, "nbmax must be >= 1");
+    if (d->n1 > 192 || d->n2 > 192 || d->n3 > 192) throw Error(AMDK_ERR_ARG, "grid dimension above 192 is not supported");
+    ctx->n1 = d->n1; ctx->n2 = d->n2; ctx->n3 = d->n3;
+    ctx->p1 = make_plan(d->n1);
+    ctx->p2 = make_plan(d->n2);
+    ctx->p3 = make_plan(d->n3);
+    ctx->prec = d->precision;
+    ctx->nbmax = d->nbmax;
+    std::memcpy(ctx->acell, d->a_cell, sizeof(ctx->acell));
+    // reciprocal: g_j = 2 pi * row j of inv(A), A columns = a_j
+    const double* a = d->a_cell;  // a[3*j+i] = (a_j)_i
+    auto A = [&](int i, int j) { return a[3 * j + i]; };
+    const double det = A(0, 0) * (A(1, 1) * A(2, 2) - A(1, 2) * A(2, 1)) - A(0, 1) * (A(1, 0) * A(2, 2) - A(1, 2) * A(2, 0)) +
+                       A(0, 2) * (A(1, 0) * A(2, 1) - A(1, 1) * A(2, 0));
+    if (!(std::fabs(det) > 1e-12)) throw Error(AMDK_ERR_ARG, "singular cell");
+    ctx->omega = std::fabs(det);
+    double inv[3][3];
+    inv[0][0] = (A(1, 1) * A(2, 2) - A(1, 2) * A(2, 1)) / det;
+    inv[0][1] = (A(0, 2) * A(2, 1) - A(0, 1) * A(2, 2)) / det;
+    inv[0][2] = (A(0, 1) * A(1, 2) - A(0, 2) * A(1, 1)) / det;
+    inv[1][0] = (A(1, 2) * A(2, 0) - A(1, 0) * A(2, 2)) / det;
+    inv[1][1] = (A(0, 0) * A(2, 2) - A(0, 2) * A(2, 0)) / det;
+    inv[1][2] = (A(0, 2) * A(1, 0) - A(0, 0) * A(1, 2)) / det;
+    inv[2][0] = (A(1, 0) * A(2, 1) - A(1, 1) * A(2, 0)) / det;
+    inv[2][1] = (A(0, 1) * A(2, 0) - A(0, 0) * A(2, 1)) / det;
+    inv[2][2] = (A(0, 0) * A(1, 1) - A(0, 1) * A(1, 0)) / det;
+    const double tpi = 2.0 * M_PI;
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) ctx->rc.g[3 * j + i] = tpi * inv[j][i];
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw Error(AMDK_ERR_DEVICE, "no HIP device available");
+    AMDK_TRY_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    build_sphere(ctx, d->miller_pw, d->ngpw, ctx->spw);
+    build_sphere(ctx, d->miller_rho, d->ngrho, ctx->srho);
+    // twiddles
+    const int ns[3] = {d->n1, d->n2, d->n3};
+    for (int ax = 0; ax < 3; ++ax) {
+      const int n = ns[ax];
+      std::vector<double2> td(n);
+      std::vector<float2> tf(n);
+      for (int t = 0; t < n; ++t) {
+        const long double ang = -2.0L * 3.141592653589793238462643383279502884L * t / n;
+        td[t].x = (double)cosl(ang);
+        td[t].y = (double)sinl(ang);
+        tf[t].x = (float)td[t].x;
+        tf[t].y = (float)td[t].y;
+      }
+      ctx->tw_d[ax] = dev_alloc(ctx, sizeof(double2) * n);
+      ctx->tw_f[ax] = dev_alloc(ctx, sizeof(float2) * n);
+      AMDK_TRY_HIP(hipMemcpy(ctx->tw_d[ax], td.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+      AMDK_TRY_HIP(hipMemcpy(ctx->tw_f[ax], tf.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+    }
+    const size_t nr = (size_t)d->n1 * d->n2 * d->n3;
+    ctx->vr = dev_alloc(ctx, (ctx->prec == AMDK_PREC_FP64 ? sizeof(double) : sizeof(float)) * nr);
+    // workspaces: pairs per chunk bounded by a 4 GiB W2 budget
+    const Sphere& S = ctx->spw;
+    const size_t w2_pair = (size_t)S.nplane * d->n2 * d->n1 * cbytes(ctx);
+    const size_t w1_pair = (size_t)S.nslot * d->n1 * cbytes(ctx);
+    const int npairs = (d->nbmax + 1) / 2;
+    const size_t budget = (size_t)4 << 30;
+    ctx->cap_pairs = (int)std::max<size_t>(1, std::min<size_t>(npairs, budget / (w2_pair + w1_pair)));
+    ctx->W1 = dev_alloc(ctx, w1_pair * ctx->cap_pairs);
+    ctx->W2 = dev_alloc(ctx, w2_pair * ctx->cap_pairs);
+    ctx->d_occ = (double*)dev_alloc(ctx, sizeof(double) * (d->nbmax + 1));
+    ctx->d_wpair = (double*)dev_alloc(ctx, sizeof(double) * (d->nbmax + 2));
+    AMDK_TRY_HIP(hipStreamSynchronize(ctx->stream));
+  } catch (const amdk::Error& e) {
+    ctx->err = e.what();
+    rc = e.code;
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    rc = AMDK_ERR_DEVICE;
+  }
+  *out = ctx;  // returned even on failure so that amdk_last_error works
+  return rc;
+}
+
+void amdk_ctx_destroy(amdk_ctx* ctx) {
+  if (!ctx) return;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (void* p : ctx->owned) (void)hipFree(p);
+  for (auto& s : ctx->pending) { (void)hipEventDestroy(s.a); (void)hipEventDestroy(s.b); }
+  for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* amdk_last_error(const amdk_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int amdk_ldc(const amdk_ctx* ctx) { return ctx ? ctx->spw.ldc : -1; }
+
+int amdk_pw_order(const amdk_ctx* ctx, int* pos, int* conj) {
+  if (!ctx || !pos || !conj) return AMDK_ERR_ARG;
+  for (int i = 0; i < ctx->spw.ncaller; ++i) {
+    pos[i] = ctx->spw.h_from_caller[i];
+    conj[i] = ctx->spw.h_conj_caller[i];
+  }
+  return AMDK_OK;
+}
+
+int amdk_set_vks(amdk_ctx* ctx, const double* v_r) {
+  API_BEGIN
+  if (!v_r) throw Error(AMDK_ERR_ARG, "null v_r");
+  const size_t nr = (size_t)ctx->n1 * ctx->n2 * ctx->n3;
+  check_finite(v_r, nr, "v_r");
+  double* stage = nullptr;
+  AMDK_TRY_HIP(hipMalloc(&stage, sizeof(double) * nr));
+  AMDK_TRY_HIP(hipMemcpyAsync(stage, v_r, sizeof(double) * nr, hipMemcpyHostToDevice, ctx->stream));
+  const double s = 1.0 / (double)nr;  // forward FFT normalisation folded into V
+  if (ctx->prec == AMDK_PREC_FP64)
+    k_scale_copy<double><<<1024, 256, 0, ctx->stream>>>(stage, (double*)ctx->vr, nr, s);
+  else
+    k_scale_copy<float><<<1024, 256, 0, ctx->stream>>>(stage, (float*)ctx->vr, nr, s);
+  AMDK_TRY_HIP(hipGetLastError());
+  AMDK_TRY_HIP(hipStreamSynchronize(ctx->stream));
+  AMDK_TRY_HIP(hipFree(stage));
+  ctx->have_vks = true;
+  API_END
+}
+
+int amdk_vpsi_dev(amdk_ctx* ctx, const void* c0, void* c2, const double* occ, int nb) {
+  API_BEGIN
+  check_nb(ctx, nb);
+  if (!c0 || !c2 || !occ) throw Error(AMDK_ERR_ARG, "null pointer");
+  if (!ctx->have_vks) throw Error(AMDK_ERR_STATE, "V_KS not set (amdk_set_vks)");
+  AMDK_TRY_HIP(hipMemcpyAsync(ctx->d_occ, occ, sizeof(double) * nb, hipMemcpyHostToDevice, ctx->stream));
+  if (ctx->prec == AMDK_PREC_FP64) run_vpsi<double>(ctx, c0, c2, nb);
+  else run_vpsi<float>(ctx, c0, c2, nb);
+  sync(ctx);
+  API_END
+}
+
+int amdk_vpsi(amdk_ctx* ctx, const double* c0, int ldc, double* c2, const double* occ, int nb) {
+  API_BEGIN
+  check_nb(ctx, nb);
+  if (!c0 || !c2 || !occ) throw Error(AMDK_ERR_ARG, "null pointer");
+  if (ldc < ctx->spw.npw) throw Error(AMDK_ERR_ARG, "ldc < ngpw");
+  if (!ctx->have_vks) throw Error(AMDK_ERR_STATE, "V_KS not set (amdk_set_vks)");
+  check_finite(c0, (size_t)2 * ldc * nb, "c0");
+  const size_t bytes = cbytes(ctx) * (size_t)ctx->spw.ldc * nb;
+  void *dc0 = nullptr, *dc2 = nullptr;
+  AMDK_TRY_HIP(hipMalloc(&dc0, bytes));
+  AMDK_TRY_HIP(hipMalloc(&dc2, bytes));
+  bands_in(ctx, c0, ldc, nb, dc0);
+  AMDK_TRY_HIP(hipMemsetAsync(dc2, 0, bytes, ctx->stream));
+  AMDK_TRY_HIP(hipMemcpyAsync(ctx->d_occ, occ, sizeof(double) * nb, hipMemcpyHostToDevice, ctx->stream));
+  if (ctx->prec == AMDK_PREC_FP64) run_vpsi<double>(ctx, dc0, dc2, nb);
+  else run_vpsi<float>(ctx, dc0, dc2, nb);
+  sync(ctx);
+  bands_out(ctx, dc2, nb, c2, ldc);
+  AMDK_TRY_HIP(hipFree(dc0));
+  AMDK_TRY_HIP(hipFree(dc2));
+  API_END
+}
+
+int amdk_density_dev(amdk_ctx* ctx, const void* c0, const double* occ, int nb, double* rho_r_dev) {
+  API_BEGIN
+  check_nb(ctx, nb);
+  if (!c0 || !occ || !rho_r_dev) throw Error(AMDK_ERR_ARG, "null pointer");
+  if (ctx->prec == AMDK_PREC_FP64) run_density<double>(ctx, c0, occ, nb, rho_r_dev);
+  else run_density<float>(ctx, c0, occ, nb, rho_r_dev);
+  sync(ctx);
+  API_END
+}
+
+int amdk_density(amdk_ctx* ctx, const double* c0, int ldc, const double* occ, int nb, double* rho_r) {
+  API_BEGIN
+  check_nb(ctx, nb);
+  if (!c0 || !occ || !rho_r) throw Error(AMDK_ERR_ARG, "null pointer");
+  if (ldc < ctx->spw.npw) throw Error(AMDK_ERR_ARG, "ldc < ngpw");
+  check_finite(c0, (size_t)2 * ldc * nb, "c0");
+  const size_t bytes = cbytes(ctx) * (size_t)ctx->spw.ldc * nb;
+  const size_t nr = (size_t)ctx->n1 * ctx->n2 * ctx->n3;
+  void* dc0 = nullptr;
+  double* drho = nullptr;
+  AMDK_TRY_HIP(hipMalloc(&dc0, bytes));
+  AMDK_TRY_HIP(hipMalloc(&drho, sizeof(double) * nr));
+  bands_in(ctx, c0, ldc, nb, dc0);
+  if (ctx->prec == AMDK_PREC_FP64) run_density<double>(ctx, dc0, occ, nb, drho);
+  else run_density<float>(ctx, dc0, occ, nb, drho);
+  sync(ctx);
+  AMDK_TRY_HIP(hipMemcpy(rho_r, drho, sizeof(double) * nr, hipMemcpyDeviceToHost));
+  AMDK_TRY_HIP(hipFree(dc0));
+  AMDK_TRY_HIP(hipFree(drho));
+  API_END
+}
+
+int amdk_bands_to_dev(amdk_ctx* ctx, const double* c, int ldc, int nb, void* dst) {
+  API_BEGIN
+  check_nb(ctx, nb);
+  if (!c || !dst) throw Error(AMDK_ERR_ARG, "null pointer");
+  if (ldc < ctx->spw.npw) throw Error(AMDK_ERR_ARG, "ldc < ngpw");
+  bands_in(ctx, c, ldc, nb, dst);
+  API_END
+}
+
+int amdk_bands_from_dev(amdk_ctx* ctx, const void* src, int nb, double* c, int ldc) {
+  API_BEGIN
+  check_nb(ctx, nb);
+  if (!c || !src) throw Error(AMDK_ERR_ARG, "null pointer");
+  if (ldc < ctx->spw.npw) throw Error(AMDK_ERR_ARG, "ldc < ngpw");
+  bands_out(ctx, src, nb, c, ldc);
+  API_END
+}
+
+int amdk_profile_enable(amdk_ctx* ctx, int on) {
+  API_BEGIN
+  ctx->prof = on != 0;
+  API_END
+}
+
+int amdk_profile_query(amdk_ctx* ctx, const char* kernel, double* total_ms, int* count) {
+  API_BEGIN
+  if (!kernel || !total_ms || !count) throw Error(AMDK_ERR_ARG, "null pointer");
+  auto it = ctx->prof_acc.find(kernel);
+  *total_ms = it == ctx->prof_acc.end() ? 0.0 : it->second.first;
+  *count = it == ctx->prof_acc.end() ? 0 : it->second.second;
+  API_END
+}
+
+int amdk_profile_reset(amdk_ctx* ctx) {
+  API_BEGIN
+  ctx->prof_acc.clear();
+  API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,86 @@
+"""ctypes binding of libamdk (include/amdk.h).
+
+This is the Python face of the same C ABI that the Fortran module binds; it
+passes plain pointers and sizes.  Loading fails loudly if the library has not
+been built: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libamdk.so")
+
+OK, ERR_ARG, ERR_DEVICE, ERR_OOM, ERR_NOT_PD, ERR_NONFINITE, ERR_STATE = range(7)
+PREC_FP64, PREC_MIXED = 0, 1
+
+# Exported symbols with (restype, argtypes).  Kept in one table so the CPU
+# test can check that every declaration in include/amdk.h is exported.
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_i = C.c_int
+
+
+class Desc(C.Structure):
+    _fields_ = [
+        ("n1", C.c_int), ("n2", C.c_int), ("n3", C.c_int),
+        ("a_cell", C.c_double * 9),
+        ("ngpw", C.c_int), ("miller_pw", _ip),
+        ("ngrho", C.c_int), ("miller_rho", _ip),
+        ("nbmax", C.c_int), ("precision", C.c_int),
+    ]
+
+
+SIGNATURES = {
+    "amdk_ctx_create": (_i, [C.POINTER(Desc), C.POINTER(_vp)]),
+    "amdk_ctx_destroy": (None, [_vp]),
+    "amdk_last_error": (C.c_char_p, [_vp]),
+    "amdk_ldc": (_i, [_vp]),
+    "amdk_pw_order": (_i, [_vp, _ip, _ip]),
+    "amdk_set_vks": (_i, [_vp, _dp]),
+    "amdk_vpsi": (_i, [_vp, _dp, _i, _dp, _dp, _i]),
+    "amdk_vpsi_dev": (_i, [_vp, _vp, _vp, _dp, _i]),
+    "amdk_density": (_i, [_vp, _dp, _i, _dp, _i, _dp]),
+    "amdk_density_dev": (_i, [_vp, _vp, _dp, _i, _vp]),
+    "amdk_bands_to_dev": (_i, [_vp, _dp, _i, _i, _vp]),
+    "amdk_bands_from_dev": (_i, [_vp, _vp, _i, _dp, _i]),
+    "amdk_profile_enable": (_i, [_vp, _i]),
+    "amdk_profile_query": (_i, [_vp, C.c_char_p, _dp, _ip]),
+    "amdk_profile_reset": (_i, [_vp]),
+}
+
+_lib = None
+
+
+def load(path=LIBPATH):
+    """Load libamdk.so and attach signatures.  Raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libamdk.so not built at {path}: run python -m amdkiit_amd.build")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+class AmdkError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"amdk error {code}: {msg}")
+        self.code = code
+
+
+def dptr(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
