@@ -43,6 +43,7 @@ void* dev_alloc(amdk_ctx* c, size_t bytes) {
 }
 
 size_t cbytes(const amdk_ctx* c) { return c->prec == AMDK_PREC_FP64 ? sizeof(double2) : sizeof(float2); }
+size_t sbytes(const amdk_ctx*) { return sizeof(double2); }
 
 int prof_begin(amdk_ctx* c, const char* name) {
   if (!c->prof) return -1;
@@ -218,12 +219,9 @@ static void bands_in(amdk_ctx* c, const double* host, int ldsrc, int nb, void* d
   double* stage = nullptr;
   AMDK_TRY_HIP(hipMalloc(&stage, sizeof(double) * std::max<size_t>(nsrc, 2)));
   AMDK_TRY_HIP(hipMemcpyAsync(stage, host, sizeof(double) * nsrc, hipMemcpyHostToDevice, c->stream));
-  AMDK_TRY_HIP(hipMemsetAsync(dst, 0, cbytes(c) * (size_t)S.ldc * nb, c->stream));
+  AMDK_TRY_HIP(hipMemsetAsync(dst, 0, sbytes(c) * (size_t)S.ldc * nb, c->stream));
   dim3 grid((S.npw + 255) / 256, nb);
-  if (c->prec == AMDK_PREC_FP64)
-    k_from_caller<double2><<<grid, 256, 0, c->stream>>>((const double2*)stage, ldsrc, (double2*)dst, S.ldc, S.d_from_caller, S.d_conj_caller, S.npw, nb);
-  else
-    k_from_caller<float2><<<grid, 256, 0, c->stream>>>((const double2*)stage, ldsrc, (float2*)dst, S.ldc, S.d_from_caller, S.d_conj_caller, S.npw, nb);
+  k_from_caller<double2><<<grid, 256, 0, c->stream>>>((const double2*)stage, ldsrc, (double2*)dst, S.ldc, S.d_from_caller, S.d_conj_caller, S.npw, nb);
   AMDK_TRY_HIP(hipGetLastError());
   AMDK_TRY_HIP(hipStreamSynchronize(c->stream));
   AMDK_TRY_HIP(hipFree(stage));
@@ -236,10 +234,7 @@ static void bands_out(amdk_ctx* c, const void* src, int nb, double* host, int ld
   AMDK_TRY_HIP(hipMalloc(&stage, sizeof(double) * std::max<size_t>(ndst, 2)));
   AMDK_TRY_HIP(hipMemsetAsync(stage, 0, sizeof(double) * ndst, c->stream));
   dim3 grid((S.npw + 255) / 256, nb);
-  if (c->prec == AMDK_PREC_FP64)
-    k_to_caller<double2><<<grid, 256, 0, c->stream>>>((const double2*)src, S.ldc, (double2*)stage, lddst, S.d_from_caller, S.d_conj_caller, S.npw, nb);
-  else
-    k_to_caller<float2><<<grid, 256, 0, c->stream>>>((const float2*)src, S.ldc, (double2*)stage, lddst, S.d_from_caller, S.d_conj_caller, S.npw, nb);
+  k_to_caller<double2><<<grid, 256, 0, c->stream>>>((const double2*)src, S.ldc, (double2*)stage, lddst, S.d_from_caller, S.d_conj_caller, S.npw, nb);
   AMDK_TRY_HIP(hipGetLastError());
   AMDK_TRY_HIP(hipMemcpyAsync(host, stage, sizeof(double) * ndst, hipMemcpyDeviceToHost, c->stream));
   AMDK_TRY_HIP(hipStreamSynchronize(c->stream));
@@ -259,7 +254,7 @@ static void launch_rows_inv(amdk_ctx* c, const Sphere& S, const void* C, int ldc
   const size_t lds = (size_t)(c->n1 + 4 * RPB * c->n1) * sizeof(V);
   dim3 grid((S.nrow + RPB - 1) / RPB, np);
   const int ps = prof_begin(c, "rows_inv");
-  k_rows_inv<T, RPB><<<grid, 256, lds, c->stream>>>(geom_of(c, S), (const V*)C, ldc, nb, pair0, (V*)c->W1, c->p1,
+  k_rows_inv<T, RPB><<<grid, 256, lds, c->stream>>>(geom_of(c, S), (const double2*)C, ldc, nb, pair0, (V*)c->W1, c->p1,
                                                      (const V*)(sizeof(T) == 4 ? c->tw_f[0] : c->tw_d[0]));
   AMDK_TRY_HIP(hipGetLastError());
   prof_end(c, ps);
@@ -305,7 +300,7 @@ static void launch_rows_fwd_hpsi(amdk_ctx* c, const Sphere& S, const void* C, vo
   const size_t lds = (size_t)(c->n1 + 4 * RPB * c->n1) * sizeof(V);
   dim3 grid((S.nrow + RPB - 1) / RPB, np);
   const int ps = prof_begin(c, "rows_fwd_hpsi");
-  k_rows_fwd_hpsi<T, RPB><<<grid, 256, lds, c->stream>>>(geom_of(c, S), (const V*)c->W1, (const V*)C, (V*)C2, S.ldc, nb,
+  k_rows_fwd_hpsi<T, RPB><<<grid, 256, lds, c->stream>>>(geom_of(c, S), (const V*)c->W1, (const double2*)C, (double2*)C2, S.ldc, nb,
                                                           pair0, c->d_occ, c->rc, c->p1,
                                                           (const V*)(sizeof(T) == 4 ? c->tw_f[0] : c->tw_d[0]));
   AMDK_TRY_HIP(hipGetLastError());
@@ -340,6 +335,9 @@ static void run_vpsi(amdk_ctx* c, const void* C0, void* C2, int nb) {
 }
 
 template <typename T>
+static void renormalise_density(amdk_ctx* c, const void* C0, const double* occ, int nb, double* rho);
+
+template <typename T>
 static void run_density(amdk_ctx* c, const void* C0, const double* occ, int nb, double* rho) {
   const Sphere& S = c->spw;
   const int npairs = (nb + 1) / 2;
@@ -353,6 +351,7 @@ static void run_density(amdk_ctx* c, const void* C0, const double* occ, int nb, 
     launch_z_rho<T>(c, S, np, rho, p0 > 0);
     AMDK_TRY_HIP(hipStreamSynchronize(c->stream));  // d_wpair reused by the next chunk
   }
+  if (sizeof(T) == 4) renormalise_density<T>(c, C0, occ, nb, rho);
 }
 
 // ---------------------------------------------------------------------------
@@ -379,6 +378,9 @@ static void check_finite(const double* p, size_t n, const char* what) {
   for (size_t i = 0; i < n; ++i)
     if (!std::isfinite(p[i])) throw Error(AMDK_ERR_NONFINITE, std::string("non-finite value in ") + what);
 }
+
+static void alloc_scf_buffers(amdk_ctx* c);
+static void destroy_scf(amdk_ctx* c);
 
 extern "C" {
 
@@ -455,6 +457,7 @@ int amdk_ctx_create(const amdk_desc* d, amdk_ctx** out) {
     ctx->W2 = dev_alloc(ctx, w2_pair * ctx->cap_pairs);
     ctx->d_occ = (double*)dev_alloc(ctx, sizeof(double) * (d->nbmax + 1));
     ctx->d_wpair = (double*)dev_alloc(ctx, sizeof(double) * (d->nbmax + 2));
+    alloc_scf_buffers(ctx);
     AMDK_TRY_HIP(hipStreamSynchronize(ctx->stream));
   } catch (const amdk::Error& e) {
     ctx->err = e.what();
@@ -470,6 +473,7 @@ int amdk_ctx_create(const amdk_desc* d, amdk_ctx** out) {
 void amdk_ctx_destroy(amdk_ctx* ctx) {
   if (!ctx) return;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  destroy_scf(ctx);
   for (void* p : ctx->owned) (void)hipFree(p);
   for (auto& s : ctx->pending) { (void)hipEventDestroy(s.a); (void)hipEventDestroy(s.b); }
   for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
@@ -529,7 +533,7 @@ int amdk_vpsi(amdk_ctx* ctx, const double* c0, int ldc, double* c2, const double
   if (ldc < ctx->spw.npw) throw Error(AMDK_ERR_ARG, "ldc < ngpw");
   if (!ctx->have_vks) throw Error(AMDK_ERR_STATE, "V_KS not set (amdk_set_vks)");
   check_finite(c0, (size_t)2 * ldc * nb, "c0");
-  const size_t bytes = cbytes(ctx) * (size_t)ctx->spw.ldc * nb;
+  const size_t bytes = sbytes(ctx) * (size_t)ctx->spw.ldc * nb;
   void *dc0 = nullptr, *dc2 = nullptr;
   AMDK_TRY_HIP(hipMalloc(&dc0, bytes));
   AMDK_TRY_HIP(hipMalloc(&dc2, bytes));
@@ -561,7 +565,7 @@ int amdk_density(amdk_ctx* ctx, const double* c0, int ldc, const double* occ, in
   if (!c0 || !occ || !rho_r) throw Error(AMDK_ERR_ARG, "null pointer");
   if (ldc < ctx->spw.npw) throw Error(AMDK_ERR_ARG, "ldc < ngpw");
   check_finite(c0, (size_t)2 * ldc * nb, "c0");
-  const size_t bytes = cbytes(ctx) * (size_t)ctx->spw.ldc * nb;
+  const size_t bytes = sbytes(ctx) * (size_t)ctx->spw.ldc * nb;
   const size_t nr = (size_t)ctx->n1 * ctx->n2 * ctx->n3;
   void* dc0 = nullptr;
   double* drho = nullptr;
@@ -610,6 +614,28 @@ int amdk_profile_query(amdk_ctx* ctx, const char* kernel, double* total_ms, int*
   API_END
 }
 
+int amdk_sphere_info(const amdk_ctx* ctx, int which, int* out6) {
+  if (!ctx || !out6 || (which != 0 && which != 1)) return AMDK_ERR_ARG;
+  const Sphere& S = which == 0 ? ctx->spw : ctx->srho;
+  out6[0] = S.npw;
+  out6[1] = S.ldc;
+  out6[2] = S.nrow;
+  out6[3] = S.nslot;
+  out6[4] = S.nplane;
+  out6[5] = S.R;
+  return AMDK_OK;
+}
+
+int amdk_profile_names(amdk_ctx* ctx, char* buf, int len) {
+  API_BEGIN
+  if (!buf || len < 1) throw Error(AMDK_ERR_ARG, "null buffer");
+  std::string all;
+  for (const auto& kv : ctx->prof_acc) all += kv.first + "\n";
+  if ((int)all.size() + 1 > len) throw Error(AMDK_ERR_ARG, "buffer too small");
+  std::memcpy(buf, all.c_str(), all.size() + 1);
+  API_END
+}
+
 int amdk_profile_reset(amdk_ctx* ctx) {
   API_BEGIN
   ctx->prof_acc.clear();
@@ -617,3 +643,5 @@ int amdk_profile_reset(amdk_ctx* ctx) {
 }
 
 }  // extern "C"
+
+#include "scf_api.inc"

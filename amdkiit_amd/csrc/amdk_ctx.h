@@ -8,6 +8,8 @@
 #include "../../include/amdk.h"
 #include "amdk_internal.h"
 #include "fft_passes.h"
+#include "gemm.h"
+#include "dense.h"
 
 #define AMDK_TRY_HIP(expr)                                                         \
   do {                                                                             \
@@ -59,6 +61,72 @@ struct amdk_ctx {
   int cap_pairs = 0;
   double* d_occ = nullptr;   // [nbmax]
   double* d_wpair = nullptr; // [nbmax+1]
+  // internal Miller indices (3 ints per G) of both spheres
+  int* d_mill_pw = nullptr;
+  int* d_mill_rho = nullptr;
+  // species tables (amdk_set_species)
+  bool have_species = false;
+  int nsp = 0, maxproj = 0;
+  std::vector<int> h_nproj, h_proj_l;
+  std::vector<double> h_dmat, h_zv;
+  double* d_vps = nullptr;    // [nsp][ldc_rho]
+  double* d_rhos = nullptr;   // [nsp][ldc_rho]
+  double* d_twnl = nullptr;   // [nsp][maxproj][ldc_pw]
+  double* d_zv = nullptr;
+  // geometry (amdk_set_positions)
+  bool have_pos = false;
+  int nat = 0, nproj_tot = 0;
+  std::vector<int> h_sp;
+  std::vector<double> h_tau;
+  std::vector<int4> h_plist;  // (atom, species, k, l)
+  int* d_sp = nullptr;
+  double* d_tau = nullptr;
+  double2* d_tab = nullptr;   // per-atom axis phases
+  int tabW = 0, hmax = 0;
+  double2* d_sv = nullptr;    // eigrxvps on the rho sphere
+  double2* d_sr = nullptr;    // eigrxrhos
+  void* d_beta = nullptr;     // [nproj_tot][ldc_pw] compute precision
+  void* d_dbeta = nullptr;
+  int4* d_plist = nullptr;
+  double* d_vpp = nullptr;    // PCG preconditioner K(G)^-1 [ldc_pw]
+  double* d_dmat = nullptr;   // [nsp][maxproj][maxproj]
+  int* d_nproj = nullptr;     // [nsp]
+  int* d_pstart = nullptr;    // [nat+1] first projector row of each atom
+  std::vector<double> h_twnl_int, h_vps0;
+  std::vector<int> h_pstart;
+  double esr = 0, eself = 0;
+  std::vector<double> h_fsr;
+  // dense FP64 workspaces (density sphere)
+  double2* W1r = nullptr;
+  double2* W2r = nullptr;
+  double* grid[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // rho, va, gx, gy, gz
+  double2* d_rhog = nullptr;  // rho(G)
+  double2* d_vg = nullptr;    // V_KS(G)
+  double2* d_fld0 = nullptr;  // 2 x ldc_rho
+  double2* d_fld1 = nullptr;
+  bool have_rhog = false;
+  // reductions and band-space matrices
+  double* d_part = nullptr;   // [8][RED_BLOCKS]
+  double* d_red = nullptr;    // [16]
+  double* d_gpart = nullptr;  // split-K partials
+  size_t gpart_cap = 0;
+  double* d_S = nullptr;      // nbmax x nbmax
+  double* d_nl = nullptr;     // P x nbmax
+  double* d_Q = nullptr;      // max(P, nbmax) x nbmax
+  double* d_finv = nullptr;   // 1/f for PCG
+  double* d_wnorm = nullptr;  // 1 for occupied bands (norm weights)
+  double* d_wdens = nullptr;  // f_b for the density norm
+  double* d_pcg = nullptr;    // PCG scalars
+  int* d_info = nullptr;
+  // resident wavefunction state for amdk_scf_step
+  void* C0 = nullptr;
+  void* C2 = nullptr;
+  void* Hd = nullptr;
+  void* Pd = nullptr;
+  void* Ct = nullptr;
+  int nb_state = 0;
+  std::vector<double> h_occ_state;
+  void* rocblas = nullptr;    // rocblas_handle
   // allocations to free
   std::vector<void*> owned;
   std::string err;
@@ -75,6 +143,7 @@ void* dev_alloc(amdk_ctx* c, size_t bytes);  // zero-filled, owned by ctx
 void sync(amdk_ctx* c);                      // stream sync + collect profile
 int prof_begin(amdk_ctx* c, const char* name);
 void prof_end(amdk_ctx* c, int slot);
-size_t cbytes(const amdk_ctx* c);            // sizeof(complex) in compute precision
+size_t cbytes(const amdk_ctx* c);            // sizeof(complex) in FFT compute precision
+size_t sbytes(const amdk_ctx* c);            // sizeof(complex) of band storage (FP64)
 
 }  // namespace amdk

@@ -1,0 +1,430 @@
+// Density-sphere (dense) transforms, exchange-correlation and the pointwise
+// pieces of the local Kohn-Sham potential, all in FP64.
+//
+// Real fields a(r), b(r) travel packed as a + i b through one complex FFT;
+// the sphere side holds their half-sphere coefficients A(G), B(G) as the two
+// "bands" of a 2 x ldc complex array, so rows_inv/cols_inv of fft_passes.h
+// serve the G->r direction unchanged.
+#pragma once
+#include "wave.h"
+
+namespace amdk {
+
+// z-FFT exp(-i) of full columns of a(r) + i b(r), keep planes |m3| <= R.
+template <int TX>
+__global__ void __launch_bounds__(256) k_z_fwd_dense(PassGeom g, const double* __restrict__ a,
+                                                     const double* __restrict__ b, double2* __restrict__ W2,
+                                                     FftPlan1D p3, const double2* __restrict__ twg) {
+  extern __shared__ unsigned char smem_raw[];
+  double2* tw = reinterpret_cast<double2*>(smem_raw);
+  double2* b0 = tw + g.n3;
+  double2* b1 = b0 + g.n3 * TX;
+  const int n1 = g.n1, n2 = g.n2, n3 = g.n3;
+  const int x0 = blockIdx.x * TX, y = blockIdx.y;
+  load_tw(tw, twg, n3);
+  for (int idx = threadIdx.x; idx < n3 * TX; idx += blockDim.x) {
+    const int z = idx / TX, tx = idx % TX, x = x0 + tx;
+    double2 v = make_double2(0, 0);
+    if (x < n1) {
+      const size_t r = ((size_t)z * n2 + y) * n1 + x;
+      v.x = a[r];
+      v.y = b ? b[r] : 0.0;
+    }
+    b0[idx] = v;
+  }
+  __syncthreads();
+  double2* res = lds_fft<double2, -1>(b0, b1, tw, p3, TX, 1, TX, true);
+  for (int idx = threadIdx.x; idx < g.nplane * TX; idx += blockDim.x) {
+    const int p = idx / TX, tx = idx % TX, x = x0 + tx;
+    if (x < n1) W2[((size_t)p * n2 + y) * n1 + x] = res[wrap(p - g.R, n3) * TX + tx];
+  }
+}
+
+// z-FFT exp(+i) of the planes |m3| <= R, full column out:
+// oa = Re, ob = Im (either may be null); ov = scale * Re in type T if non-null.
+template <typename T, int TX>
+__global__ void __launch_bounds__(256) k_z_inv_dense(PassGeom g, const double2* __restrict__ W2,
+                                                     double* __restrict__ oa, double* __restrict__ ob,
+                                                     T* __restrict__ ov, double scale, FftPlan1D p3,
+                                                     const double2* __restrict__ twg) {
+  extern __shared__ unsigned char smem_raw[];
+  double2* tw = reinterpret_cast<double2*>(smem_raw);
+  double2* b0 = tw + g.n3;
+  double2* b1 = b0 + g.n3 * TX;
+  const int n1 = g.n1, n2 = g.n2, n3 = g.n3;
+  const int x0 = blockIdx.x * TX, y = blockIdx.y;
+  load_tw(tw, twg, n3);
+  for (int i = threadIdx.x; i < n3 * TX; i += blockDim.x) b0[i] = make_double2(0, 0);
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < g.nplane * TX; idx += blockDim.x) {
+    const int p = idx / TX, tx = idx % TX, x = x0 + tx;
+    if (x < n1) b0[wrap(p - g.R, n3) * TX + tx] = W2[((size_t)p * n2 + y) * n1 + x];
+  }
+  __syncthreads();
+  double2* res = lds_fft<double2, +1>(b0, b1, tw, p3, TX, 1, TX, true);
+  for (int idx = threadIdx.x; idx < n3 * TX; idx += blockDim.x) {
+    const int z = idx / TX, tx = idx % TX, x = x0 + tx;
+    if (x < n1) {
+      const size_t r = ((size_t)z * n2 + y) * n1 + x;
+      if (oa) oa[r] = res[idx].x;
+      if (ob) ob[r] = res[idx].y;
+      if (ov) ov[r] = (T)(scale * res[idx].x);
+    }
+  }
+}
+
+// x-FFT exp(-i) of the rows and unpack two real fields:
+// A = (F(G) + conj F(-G)) / 2, B = (F(G) - conj F(-G)) / 2i, times `scale`.
+template <int RPB>
+__global__ void __launch_bounds__(256) k_rows_fwd_unpack(PassGeom g, const double2* __restrict__ W1,
+                                                         double2* __restrict__ out, int ldc, double scale,
+                                                         FftPlan1D p1, const double2* __restrict__ twg) {
+  extern __shared__ unsigned char smem_raw[];
+  double2* tw = reinterpret_cast<double2*>(smem_raw);
+  double2* b0 = tw + g.n1;
+  double2* b1 = b0 + 2 * RPB * g.n1;
+  const int n1 = g.n1;
+  const int r0 = blockIdx.x * RPB;
+  const int nr = min(RPB, g.nrow - r0);
+  load_tw(tw, twg, n1);
+  for (int lr = 0; lr < nr; ++lr) {
+    const RowInfo ri = g.rows[r0 + lr];
+    const double2* id = W1 + (size_t)ri.slot_d * n1;
+    for (int i = threadIdx.x; i < n1; i += blockDim.x) b0[(2 * lr) * n1 + i] = id[i];
+    if (ri.slot_m >= 0) {
+      const double2* im = W1 + (size_t)ri.slot_m * n1;
+      for (int i = threadIdx.x; i < n1; i += blockDim.x) b0[(2 * lr + 1) * n1 + i] = im[i];
+    } else {
+      for (int i = threadIdx.x; i < n1; i += blockDim.x) b0[(2 * lr + 1) * n1 + i] = make_double2(0, 0);
+    }
+  }
+  __syncthreads();
+  double2* res = lds_fft<double2, -1>(b0, b1, tw, p1, 2 * nr, n1, 1, false);
+  for (int lr = 0; lr < nr; ++lr) {
+    const RowInfo ri = g.rows[r0 + lr];
+    const double2* rd = res + (2 * lr) * n1;
+    const double2* rm = (ri.slot_m >= 0) ? rd + n1 : rd;
+    for (int e = threadIdx.x; e < ri.cnt; e += blockDim.x) {
+      const int m1 = ri.m1_0 + e;
+      const double2 P = rd[wrap(m1, n1)], M = rm[wrap(-m1, n1)];
+      const double2 fa = make_double2(P.x + M.x, P.y + M.y), fb = make_double2(P.x - M.x, P.y - M.y);
+      out[ri.start + e] = make_double2(0.5 * scale * fa.x, 0.5 * scale * fb.y);
+      out[(size_t)ldc + ri.start + e] = make_double2(0.5 * scale * fa.y, -0.5 * scale * fb.x);
+    }
+  }
+}
+
+__device__ __forceinline__ void gvec_of(const int* mill, int g, const Recip& rc, double& gx, double& gy, double& gz) {
+  const double m1 = mill[3 * g], m2 = mill[3 * g + 1], m3 = mill[3 * g + 2];
+  gx = m1 * rc.g[0] + m2 * rc.g[3] + m3 * rc.g[6];
+  gy = m1 * rc.g[1] + m2 * rc.g[4] + m3 * rc.g[7];
+  gz = m1 * rc.g[2] + m2 * rc.g[5] + m3 * rc.g[8];
+}
+
+// Hartree + local: vg = 4 pi/G^2 (rho + rho_I) + V_ps (G != 0), vg(0) = 0
+// (potential.f90:4-46) and the partial sums of E_H and E_loc
+// (total_energy.f90:64-131, pseudopotential.f90:22-34).
+__global__ void __launch_bounds__(256) k_hartree(const int* __restrict__ mill, int ng, Recip rc,
+                                                 const double2* __restrict__ rhog, const double2* __restrict__ sr,
+                                                 const double2* __restrict__ sv, double2* __restrict__ vg,
+                                                 double* __restrict__ part) {
+  __shared__ double sh[4];
+  double eh = 0, el = 0;
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x) {
+    double gx, gy, gz;
+    gvec_of(mill, g, rc, gx, gy, gz);
+    const double g2 = gx * gx + gy * gy + gz * gz;
+    const double2 r = rhog[g], ri = sr[g], v = sv[g];
+    if (g2 > 1e-12) {
+      const double f = 4.0 * M_PI / g2;
+      const double2 rt = make_double2(r.x + ri.x, r.y + ri.y);
+      eh += f * (rt.x * rt.x + rt.y * rt.y);
+      el += v.x * r.x + v.y * r.y;
+      vg[g] = make_double2(f * rt.x + v.x, f * rt.y + v.y);
+    } else {
+      el += 0.5 * (v.x * r.x + v.y * r.y);
+      vg[g] = make_double2(0, 0);
+    }
+  }
+  const double a = block_sum(eh, sh);
+  const double b = block_sum(el, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[RED_BLOCKS + blockIdx.x] = b;
+  }
+}
+
+// fields for grad rho: pair 0 = (i Gx rho, i Gy rho), pair 1 = (i Gz rho, 0)
+__global__ void k_grad_fields(const int* __restrict__ mill, int ng, int ldc, Recip rc,
+                              const double2* __restrict__ rhog, double2* __restrict__ f0,
+                              double2* __restrict__ f1) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ldc; g += gridDim.x * blockDim.x) {
+    if (g < ng) {
+      double gx, gy, gz;
+      gvec_of(mill, g, rc, gx, gy, gz);
+      const double2 r = rhog[g];
+      f0[g] = make_double2(-gx * r.y, gx * r.x);
+      f0[ldc + g] = make_double2(-gy * r.y, gy * r.x);
+      f1[g] = make_double2(-gz * r.y, gz * r.x);
+      f1[ldc + g] = make_double2(0, 0);
+    } else {
+      f0[g] = f0[ldc + g] = f1[g] = f1[ldc + g] = make_double2(0, 0);
+    }
+  }
+}
+
+// vg += A1 - i Gx B1 - i Gy A2 - i Gz B2  (xc.f90:264-296); A2/B2 may be null (LDA)
+__global__ void k_vassemble(const int* __restrict__ mill, int ng, int ldc, Recip rc,
+                            const double2* __restrict__ p1, const double2* __restrict__ p2,
+                            double2* __restrict__ vg) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x) {
+    double gx, gy, gz;
+    gvec_of(mill, g, rc, gx, gy, gz);
+    const double2 A1 = p1[g], B1 = p1[ldc + g];
+    double2 v = vg[g];
+    v.x += A1.x + gx * B1.y;
+    v.y += A1.y - gx * B1.x;
+    if (p2) {
+      const double2 A2 = p2[g], B2 = p2[ldc + g];
+      v.x += gy * A2.y + gz * B2.y;
+      v.y += -gy * A2.x - gz * B2.x;
+    }
+    vg[g] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// exchange-correlation energy density f = rho eps and df/drho, df/dsigma
+enum { XC_TETER93 = 0, XC_PZ = 1, XC_PBE = 2 };
+
+__device__ __forceinline__ void xc_teter93(double rho, double& f, double& vr) {
+  const double a0 = 0.4581652932831429, a1 = 2.217058676663745, a2 = 0.7405551735357053, a3 = 0.01968227878617998;
+  const double b1 = 1.0, b2 = 4.504130959426697, b3 = 1.110667363742916, b4 = 0.02359291751427506;
+  const double rs = cbrt(3.0 / (4.0 * M_PI * rho));
+  const double n = a0 + rs * (a1 + rs * (a2 + rs * a3));
+  const double d = rs * (b1 + rs * (b2 + rs * (b3 + rs * b4)));
+  const double dn = a1 + rs * (2 * a2 + rs * 3 * a3);
+  const double dd = b1 + rs * (2 * b2 + rs * (3 * b3 + rs * 4 * b4));
+  const double eps = -n / d;
+  const double deps = -(dn * d - n * dd) / (d * d);
+  f = rho * eps;
+  vr = eps - rs / 3.0 * deps;
+}
+
+__device__ __forceinline__ void xc_pz81(double rho, double& f, double& vr) {
+  const double cx = -0.75 * cbrt(3.0 / M_PI);
+  const double r13 = cbrt(rho);
+  const double ex = cx * r13;
+  double ec, vc;
+  const double rs = cbrt(3.0 / (4.0 * M_PI * rho));
+  if (rs >= 1.0) {
+    const double g = -0.1423, b1 = 1.0529, b2 = 0.3334;
+    const double sq = sqrt(rs);
+    const double den = 1.0 + b1 * sq + b2 * rs;
+    ec = g / den;
+    vc = ec * (1.0 + 7.0 / 6.0 * b1 * sq + 4.0 / 3.0 * b2 * rs) / den;
+  } else {
+    const double A = 0.0311, B = -0.048, C = 0.0020, D = -0.0116;
+    const double lr = log(rs);
+    ec = A * lr + B + C * rs * lr + D * rs;
+    vc = A * lr + (B - A / 3.0) + 2.0 / 3.0 * C * rs * lr + (2.0 * D - C) / 3.0 * rs;
+  }
+  f = rho * (ex + ec);
+  vr = 4.0 / 3.0 * ex + vc;
+}
+
+__device__ __forceinline__ void xc_pbe(double rho, double sigma, double& f, double& vr, double& vs) {
+  // exchange
+  const double kappa = 0.804, mu = 0.2195149727645171;
+  const double cx = -0.75 * cbrt(3.0 / M_PI);
+  const double r13 = cbrt(rho);
+  const double ex = cx * rho * r13;                           // rho * eps_x^LDA
+  const double cs = 1.0 / (4.0 * cbrt(9.0 * M_PI * M_PI * M_PI * M_PI));  // s^2 = cs sigma rho^{-8/3}
+  const double rm83 = 1.0 / (rho * rho * r13 * r13);
+  const double s2 = cs * sigma * rm83;
+  const double den = 1.0 + mu * s2 / kappa;
+  const double Fx = 1.0 + kappa - kappa / den;
+  const double dF = mu / (den * den);                         // dFx/ds2
+  const double fx = ex * Fx;
+  const double vrx = 4.0 / 3.0 * (ex / rho) * Fx - ex * dF * (8.0 / 3.0) * s2 / rho;
+  const double vsx = ex * dF * cs * rm83;
+  // correlation: PW92 (PW_MOD constants) + H
+  const double A = 0.0310907, a1 = 0.21370, B1 = 7.5957, B2 = 3.5876, B3 = 1.6382, B4 = 0.49294;
+  const double beta = 0.06672455060314922, gamma = (1.0 - M_LN2) / (M_PI * M_PI);
+  const double rs = cbrt(3.0 / (4.0 * M_PI * rho));
+  const double srs = sqrt(rs);
+  const double Q = 2.0 * A * (B1 * srs + B2 * rs + B3 * rs * srs + B4 * rs * rs);
+  const double dQ = 2.0 * A * (0.5 * B1 / srs + B2 + 1.5 * B3 * srs + 2.0 * B4 * rs);
+  const double lq = log(1.0 + 1.0 / Q);
+  const double ec = -2.0 * A * (1.0 + a1 * rs) * lq;
+  const double dec = -2.0 * A * a1 * lq + 2.0 * A * (1.0 + a1 * rs) * dQ / (Q * Q + Q);
+  const double ct = M_PI / (16.0 * cbrt(3.0 * M_PI * M_PI));   // t^2 = ct sigma rho^{-7/3}
+  const double rm73 = 1.0 / (rho * rho * r13);
+  const double t2 = ct * sigma * rm73;
+  const double ee = exp(-ec / gamma);
+  const double Ac = (beta / gamma) / (ee - 1.0);
+  const double dAde = Ac * Ac * ee / beta;
+  const double y = Ac * t2;
+  const double N = 1.0 + y, Dn = 1.0 + y + y * y;
+  const double R = t2 * N / Dn;
+  const double arg = 1.0 + (beta / gamma) * R;
+  const double H = gamma * log(arg);
+  const double dRdt2 = N / Dn - y * y * (2.0 + y) / (Dn * Dn);
+  const double dRdA = -t2 * t2 * y * (2.0 + y) / (Dn * Dn);
+  const double dHdt2 = beta * dRdt2 / arg;
+  const double dHdA = beta * dRdA / arg;
+  const double drs = -rs / (3.0 * rho);
+  const double fc = rho * (ec + H);
+  const double vrc = ec + H + rho * (dec * drs + dHdA * dAde * dec * drs + dHdt2 * (-7.0 / 3.0) * t2 / rho);
+  const double vsc = rho * dHdt2 * ct * rm73;
+  f = fx + fc;
+  vr = vrx + vrc;
+  vs = vsx + vsc;
+}
+
+// XC on the grid with the reference's low-density smoothing (xc.f90:100-133).
+// In: rho, grad components (GGA only).  Out: v1 -> va; v2*grad -> gx,gy,gz in
+// place; partial E_xc sums (times Omega/N later).
+__global__ void __launch_bounds__(256) k_xc(int kind, size_t nr, const double* __restrict__ rho,
+                                            double* __restrict__ va, double* __restrict__ gx,
+                                            double* __restrict__ gy, double* __restrict__ gz,
+                                            double* __restrict__ part) {
+  __shared__ double sh[4];
+  const double gc = 1.0e-8;
+  double exc = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nr; i += (size_t)gridDim.x * blockDim.x) {
+    const double r = fmax(rho[i], 0.0);
+    double v1 = 0, v2 = 0;
+    if (r > 0.1 * gc) {
+      double smoo = 1.0, dsmoo = 0.0;
+      if (r <= 4.0 * gc) {
+        const double texp = exp(3.0 * (1.0 - r / gc));
+        smoo = 1.0 / (1.0 + texp);
+        dsmoo = 3.0 / gc * texp * smoo * smoo;
+      }
+      double f = 0, vr = 0, vs = 0;
+      if (kind == XC_TETER93) xc_teter93(r, f, vr);
+      else if (kind == XC_PZ) xc_pz81(r, f, vr);
+      else {
+        const double s = gx[i] * gx[i] + gy[i] * gy[i] + gz[i] * gz[i];
+        xc_pbe(r, fmax(s, 1e-60), f, vr, vs);
+      }
+      exc += smoo * f;
+      v1 = dsmoo * f + smoo * vr;
+      v2 = smoo * 2.0 * vs;
+    }
+    va[i] = v1;
+    if (kind == XC_PBE) {
+      gx[i] *= v2;
+      gy[i] *= v2;
+      gz[i] *= v2;
+    }
+  }
+  const double e = block_sum(exc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = e;
+}
+
+// sum of a real grid (for the electron count)
+__global__ void __launch_bounds__(256) k_grid_sum(const double* __restrict__ x, size_t n, double* __restrict__ part) {
+  __shared__ double sh[4];
+  double s = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) s += x[i];
+  const double r = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// E_kin partial: sum_b f_b sum_G |G|^2 |c|^2 (half sphere, total_energy.f90:38-61)
+template <typename V>
+__global__ void __launch_bounds__(256) k_ekin(const V* __restrict__ C, int ldc, int npw, int nb,
+                                              const int* __restrict__ mill, Recip rc,
+                                              const double* __restrict__ occ, double* __restrict__ part) {
+  __shared__ double sh[4];
+  double s = 0;
+  const size_t tot = (size_t)nb * npw;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+    const int b = (int)(i / npw), g = (int)(i % npw);
+    double gx, gy, gz;
+    gvec_of(mill, g, rc, gx, gy, gz);
+    const V c = C[(size_t)b * ldc + g];
+    s += occ[b] * (gx * gx + gy * gy + gz * gz) * ((double)c.x * c.x + (double)c.y * c.y);
+  }
+  const double r = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// Local/Hartree force POTFOR (nuclear_grad.f90:4-50), one block per atom:
+// F_a = 2 Omega sum_{G != 0} Re[i G e^{-iGR_a} (rho_s 4pi/G^2 conj(rho_tot) + V_s conj(rho_e))]
+__global__ void __launch_bounds__(256) k_potfor(const int* __restrict__ mill, int ng, Recip rc,
+                                                const int* __restrict__ sp, const double* __restrict__ vps,
+                                                const double* __restrict__ rhos, int ldg,
+                                                const double2* __restrict__ rhog, const double2* __restrict__ sr,
+                                                const double2* __restrict__ tab, int W, int hmax, double omega,
+                                                double* __restrict__ force) {
+  __shared__ double sh[4];
+  const int a = blockIdx.x;
+  const int s = sp[a];
+  double f0 = 0, f1 = 0, f2 = 0;
+  for (int g = threadIdx.x + 1; g < ng; g += blockDim.x) {
+    double gx, gy, gz;
+    gvec_of(mill, g, rc, gx, gy, gz);
+    const double g2 = gx * gx + gy * gy + gz * gz;
+    const double2 re = rhog[g], ri = sr[g];
+    const double2 rtc = make_double2(re.x + ri.x, -(re.y + ri.y));  // conj(rho_tot)
+    const double fh = 4.0 * M_PI / g2;
+    const double rs_ = rhos[(size_t)s * ldg + g], vs_ = vps[(size_t)s * ldg + g];
+    const double2 t = make_double2(rs_ * fh * rtc.x + vs_ * re.x, rs_ * fh * rtc.y - vs_ * re.y);
+    const double2 e = eigr_of(tab, a, W, hmax, mill[3 * g], mill[3 * g + 1], mill[3 * g + 2]);
+    const double2 et = zmul(e, t);
+    // Re(i G et) = -G Im(et)
+    f0 += -gx * et.y;
+    f1 += -gy * et.y;
+    f2 += -gz * et.y;
+  }
+  const double r0 = block_sum(f0, sh);
+  const double r1 = block_sum(f1, sh);
+  const double r2 = block_sum(f2, sh);
+  if (threadIdx.x == 0) {
+    force[3 * a] = 2.0 * omega * r0;
+    force[3 * a + 1] = 2.0 * omega * r1;
+    force[3 * a + 2] = 2.0 * omega * r2;
+  }
+}
+
+// Short-range ion-ion energy and forces, minimum image for cubic cells
+// (e_sr, total_energy.f90:133-239).  One thread per atom i, ordered sum over j;
+// energy counted for j > i.
+__global__ void k_esr(const double* __restrict__ tau, int nat, const int* __restrict__ sp,
+                      const double* __restrict__ zv, double L, int cubic, double rckj,
+                      double* __restrict__ force, double* __restrict__ epart) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nat) return;
+  double e = 0, fx = 0, fy = 0, fz = 0;
+  for (int j = 0; j < nat; ++j) {
+    if (j == i) continue;
+    double d[3] = {tau[3 * i] - tau[3 * j], tau[3 * i + 1] - tau[3 * j + 1], tau[3 * i + 2] - tau[3 * j + 2]};
+    if (cubic) {
+      for (int k = 0; k < 3; ++k) {
+        double x = fmod(d[k], L);
+        d[k] = x - L * trunc(x / (0.5 * L));
+      }
+    }
+    const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const double r = sqrt(r2);
+    const double arg = r / rckj;
+    if (arg > 20.0) continue;
+    const double z2 = zv[sp[i]] * zv[sp[j]];
+    const double add = z2 * erfc(arg) / r;
+    if (j > i) e += add;
+    const double pre = 2.0 * z2 / sqrt(M_PI) * exp(-arg * arg) / rckj;
+    const double rep = (add + pre) / r2;
+    fx += rep * d[0];
+    fy += rep * d[1];
+    fz += rep * d[2];
+  }
+  force[3 * i] = fx;
+  force[3 * i + 1] = fy;
+  force[3 * i + 2] = fz;
+  epart[i] = e;
+}
+
+}  // namespace amdk

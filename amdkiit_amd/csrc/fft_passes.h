@@ -42,7 +42,7 @@ __device__ __forceinline__ void load_tw(V* tw, const V* __restrict__ twg, int n)
 // psi1 + i psi2 on the direct row and conj(psi1) + i conj(psi2) on the mirror
 // row, then x-FFT with exp(+i).
 template <typename T, int RPB>
-__global__ void __launch_bounds__(256) k_rows_inv(PassGeom g, const typename cplx_of<T>::type* __restrict__ C,
+__global__ void __launch_bounds__(256) k_rows_inv(PassGeom g, const double2* __restrict__ C,
                                                   int ldc, int nb, int pair0,
                                                   typename cplx_of<T>::type* __restrict__ W1, FftPlan1D p1,
                                                   const typename cplx_of<T>::type* __restrict__ twg) {
@@ -66,9 +66,18 @@ __global__ void __launch_bounds__(256) k_rows_inv(PassGeom g, const typename cpl
     V* rd = b0 + (2 * lr) * n1;
     V* rm = (ri.slot_m >= 0) ? rd + n1 : rd;
     for (int e = threadIdx.x; e < ri.cnt; e += blockDim.x) {
-      const V c1 = C[(size_t)ba * ldc + ri.start + e];
-      V c2;
-      if (has2) c2 = C[(size_t)bb * ldc + ri.start + e]; else { c2.x = 0; c2.y = 0; }
+      const double2 s1 = C[(size_t)ba * ldc + ri.start + e];
+      V c1, c2;
+      c1.x = (T)s1.x;
+      c1.y = (T)s1.y;
+      if (has2) {
+        const double2 s2 = C[(size_t)bb * ldc + ri.start + e];
+        c2.x = (T)s2.x;
+        c2.y = (T)s2.y;
+      } else {
+        c2.x = 0;
+        c2.y = 0;
+      }
       const int m1 = ri.m1_0 + e;
       V d, m;
       d.x = c1.x - c2.y;
@@ -269,8 +278,8 @@ __global__ void __launch_bounds__(256) k_z_rho(PassGeom g, const typename cplx_o
 // two bands and write C2 = -(f/2) (|G|^2 c + 2 (V psi)(G)) (f = 0 -> factor 1).
 template <typename T, int RPB>
 __global__ void __launch_bounds__(256) k_rows_fwd_hpsi(PassGeom g, const typename cplx_of<T>::type* __restrict__ W1,
-                                                       const typename cplx_of<T>::type* __restrict__ C,
-                                                       typename cplx_of<T>::type* __restrict__ C2, int ldc,
+                                                       const double2* __restrict__ C,
+                                                       double2* __restrict__ C2, int ldc,
                                                        int nb, int pair0, const double* __restrict__ occ,
                                                        Recip rc, FftPlan1D p1,
                                                        const typename cplx_of<T>::type* __restrict__ twg) {
@@ -304,7 +313,7 @@ __global__ void __launch_bounds__(256) k_rows_fwd_hpsi(PassGeom g, const typenam
   if (fa == 0.0) fa = 1.0;
   double fb = has2 ? occ[bb] * 0.5 : 1.0;
   if (fb == 0.0) fb = 1.0;
-  const T fia = (T)fa, fib = (T)fb;
+
   for (int lr = 0; lr < nr; ++lr) {
     const RowInfo ri = g.rows[r0 + lr];
     const V* rd = res + (2 * lr) * n1;
@@ -315,23 +324,17 @@ __global__ void __launch_bounds__(256) k_rows_fwd_hpsi(PassGeom g, const typenam
     for (int e = threadIdx.x; e < ri.cnt; e += blockDim.x) {
       const int m1 = ri.m1_0 + e;
       const double gx = m1 * rc.g[0] + gy0, gyv = m1 * rc.g[1] + gy1, gz = m1 * rc.g[2] + gy2;
-      const T gg = (T)(gx * gx + gyv * gyv + gz * gz);
+      const double gg = gx * gx + gyv * gyv + gz * gz;
       const V A = rd[wrap(m1, n1)];
       const V B = rm[wrap(-m1, n1)];
       const V FP = cadd(A, B), FM = csub(A, B);
       const size_t ia = (size_t)ba * ldc + ri.start + e;
-      const V c1 = C[ia];
-      V o1;
-      o1.x = -fia * (gg * c1.x + FP.x);
-      o1.y = -fia * (gg * c1.y + FM.y);
-      C2[ia] = o1;
+      const double2 c1 = C[ia];
+      C2[ia] = make_double2(-fa * (gg * c1.x + (double)FP.x), -fa * (gg * c1.y + (double)FM.y));
       if (has2) {
         const size_t ib = (size_t)bb * ldc + ri.start + e;
-        const V c2 = C[ib];
-        V o2;
-        o2.x = -fib * (gg * c2.x + FP.y);
-        o2.y = -fib * (gg * c2.y - FM.x);
-        C2[ib] = o2;
+        const double2 c2 = C[ib];
+        C2[ib] = make_double2(-fb * (gg * c2.x + (double)FP.y), -fb * (gg * c2.y - (double)FM.x));
       }
     }
   }

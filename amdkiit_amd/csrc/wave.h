@@ -1,0 +1,243 @@
+// Wavefunction-side kernels: structure factors and projector tables for a
+// geometry, the Gamma-point clean-up, deterministic FP64 reductions, the
+// preconditioned descent of PCG, and the real-space/G-space pointwise pieces
+// that do not belong to an FFT pass.
+#pragma once
+#include "fft_core.h"
+
+namespace amdk {
+
+// ---------------------------------------------------------------------------
+// block reductions (256 threads)
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x / 64); ++i) r += sh[i];
+  __syncthreads();
+  return r;  // valid in thread 0
+}
+__device__ __forceinline__ double block_max(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double r = -1.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x / 64); ++i) r = fmax(r, sh[i]);
+  __syncthreads();
+  return r;
+}
+
+constexpr int RED_BLOCKS = 512;
+
+// Final pass: out[q] = sum (or max) over RED_BLOCKS partials of quantity q.
+__global__ void k_red_final(const double* __restrict__ part, int nq, double* __restrict__ out, int is_max) {
+  __shared__ double sh[4];
+  for (int q = 0; q < nq; ++q) {
+    double v = is_max ? -1.0 : 0.0;
+    for (int i = threadIdx.x; i < RED_BLOCKS; i += blockDim.x) {
+      const double x = part[(size_t)q * RED_BLOCKS + i];
+      v = is_max ? fmax(v, x) : v + x;
+    }
+    const double r = is_max ? block_max(v, sh) : block_sum(v, sh);
+    if (threadIdx.x == 0) out[q] = r;
+  }
+}
+
+// sum_b w_b * DOTP(X_b, X_b) (full-sphere norm from the half sphere, G=0 once)
+// and max |X_b(G)| over b < nmax: part[0][blk] = sum, part[1][blk] = max.
+template <typename V>
+__global__ void __launch_bounds__(256) k_norms(const V* __restrict__ X, int ldx, int npw, int nb,
+                                               const double* __restrict__ w, double* __restrict__ part) {
+  __shared__ double sh[4];
+  double s = 0, mx = 0;
+  const size_t tot = (size_t)nb * npw;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+    const int b = (int)(i / npw), g = (int)(i % npw);
+    const V x = X[(size_t)b * ldx + g];
+    const double a2 = (double)x.x * x.x + (double)x.y * x.y;
+    s += w[b] * (g == 0 ? (double)x.x * x.x : 2.0 * a2);
+    mx = fmax(mx, sqrt(a2));
+  }
+  const double rs = block_sum(s, sh);
+  const double rm = block_max(mx, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = rs;
+    part[RED_BLOCKS + blockIdx.x] = rm;
+  }
+}
+
+template <typename V>
+__global__ void k_zclean(V* __restrict__ C, int ldc, int nb) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb) C[(size_t)b * ldc].y = 0;
+}
+
+// ---------------------------------------------------------------------------
+// structure factors (exp_igrxfactor): per-atom axis phases e^{-i m s_k},
+// s_k = g_k . R_a, for m in [-h, h], h = n_k/2.
+__global__ void k_phase_tables(const double* __restrict__ tau, int nat, const double* __restrict__ gm,
+                               int n1, int n2, int n3, double2* __restrict__ tab) {
+  // tab layout: [a][axis][m + hmax], hmax = max(n)/2, width W = 2*hmax+1
+  const int hmax = max(n1, max(n2, n3)) / 2, W = 2 * hmax + 1;
+  const int tot = nat * 3 * W;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += gridDim.x * blockDim.x) {
+    const int m = i % W - hmax;
+    const int ax = (i / W) % 3;
+    const int a = i / (3 * W);
+    const double s = gm[3 * ax] * tau[3 * a] + gm[3 * ax + 1] * tau[3 * a + 1] + gm[3 * ax + 2] * tau[3 * a + 2];
+    double sn, cs;
+    sincos(-(double)m * s, &sn, &cs);
+    tab[i] = make_double2(cs, sn);
+  }
+}
+
+__device__ __forceinline__ double2 zmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ double2 eigr_of(const double2* tab, int a, int W, int hmax, int m1, int m2, int m3) {
+  const double2* t = tab + (size_t)a * 3 * W;
+  return zmul(zmul(t[m1 + hmax], t[W + m2 + hmax]), t[2 * W + m3 + hmax]);
+}
+
+// eigrxvps(G) = sum_a V_s(G) e^{-iGR_a};  eigrxrhos(G) = sum_a rho_s(G) e^{-iGR_a}
+__global__ void k_strfac(const int* __restrict__ mill, int ng, int nat, const int* __restrict__ sp,
+                         const double* __restrict__ vps, const double* __restrict__ rhos, int ldg,
+                         const double2* __restrict__ tab, int W, int hmax, double2* __restrict__ sv,
+                         double2* __restrict__ sr) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x) {
+    const int m1 = mill[3 * g], m2 = mill[3 * g + 1], m3 = mill[3 * g + 2];
+    double2 av = make_double2(0, 0), ar = make_double2(0, 0);
+    for (int a = 0; a < nat; ++a) {
+      const double2 e = eigr_of(tab, a, W, hmax, m1, m2, m3);
+      const double v = vps[(size_t)sp[a] * ldg + g], r = rhos[(size_t)sp[a] * ldg + g];
+      av.x += v * e.x; av.y += v * e.y;
+      ar.x += r * e.x; ar.y += r * e.y;
+    }
+    sv[g] = av;
+    sr[g] = ar;
+  }
+}
+
+// beta_p(G) = (-i)^l twnl_{s,k}(G) e^{-iGR_a} for projector p = (a, s, k, l)
+template <typename V>
+__global__ void k_beta(const int* __restrict__ mill, int npw, int ldc, const int4* __restrict__ plist, int np,
+                       const double* __restrict__ twnl, int maxproj, const double2* __restrict__ tab, int W,
+                       int hmax, V* __restrict__ beta) {
+  const int p = blockIdx.y;
+  const int4 pi = plist[p];  // a, s, k, l
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ldc; g += gridDim.x * blockDim.x) {
+    V o;
+    if (g < npw) {
+      const double2 e = eigr_of(tab, pi.x, W, hmax, mill[3 * g], mill[3 * g + 1], mill[3 * g + 2]);
+      const double t = twnl[((size_t)pi.y * maxproj + pi.z) * ldc + g];
+      double2 b = make_double2(t * e.x, t * e.y);
+      switch (pi.w & 3) {  // multiply by (-i)^l
+        case 1: b = make_double2(b.y, -b.x); break;
+        case 2: b = make_double2(-b.x, -b.y); break;
+        case 3: b = make_double2(-b.y, b.x); break;
+        default: break;
+      }
+      o.x = b.x;
+      o.y = b.y;
+    } else {
+      o.x = 0;
+      o.y = 0;
+    }
+    beta[(size_t)p * ldc + g] = o;
+  }
+}
+
+// derivative projectors: out_p(G) = -i G_dir beta_p(G)
+template <typename V>
+__global__ void k_dbeta(const V* __restrict__ beta, const int* __restrict__ mill, int npw, int ldc, int np,
+                        Recip rc, int dir, V* __restrict__ out) {
+  const int p = blockIdx.y;
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ldc; g += gridDim.x * blockDim.x) {
+    V o;
+    o.x = 0;
+    o.y = 0;
+    if (g < npw) {
+      const double gk = mill[3 * g] * rc.g[dir] + mill[3 * g + 1] * rc.g[3 + dir] + mill[3 * g + 2] * rc.g[6 + dir];
+      const V b = beta[(size_t)p * ldc + g];
+      o.x = (double)b.y * gk;
+      o.y = -(double)b.x * gk;
+    }
+    out[(size_t)p * ldc + g] = o;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// PCG (cg.f90:63-145).  P = f^-1 K(G)^-1 C2, ||P||^2 reduced, then
+// d = P + gamma d_prev and C0 += step d.
+template <typename V>
+__global__ void k_precon(const V* __restrict__ C2, V* __restrict__ P, int ldc, int npw, int nb,
+                         const double* __restrict__ vpp, const double* __restrict__ finv) {
+  const int b = blockIdx.y;
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ldc; g += gridDim.x * blockDim.x) {
+    const size_t i = (size_t)b * ldc + g;
+    V o;
+    if (g < npw) {
+      const double s = finv[b] * vpp[g];
+      o.x = s * (double)C2[i].x;
+      o.y = s * (double)C2[i].y;
+    } else {
+      o.x = 0;
+      o.y = 0;
+    }
+    P[i] = o;
+  }
+}
+
+// scalars: st[0] = ||P||^2 of this step (input), st[1] = previous ||P||^2,
+// st[2] = gamma out.  istep is 1-based within one optimisation.
+__global__ void k_pcg_scalar(double* st, int istep) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double gamma = 0.0;
+    if (istep > 1) {
+      const double forget = istep <= 10 ? 0.60 : (istep <= 20 ? 0.91 : 0.95);
+      gamma = forget * st[0] / st[1];
+    }
+    st[2] = gamma;
+    st[1] = st[0];
+  }
+}
+
+// d = P + gamma d ; C0 += step d   (d stored in H, P read from P)
+template <typename V>
+__global__ void k_pcg_update(V* __restrict__ C0, V* __restrict__ H, const V* __restrict__ P, size_t n,
+                             const double* __restrict__ st, double step) {
+  const double gamma = st[2];
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    V d;
+    d.x = (double)P[i].x + gamma * (double)H[i].x;
+    d.y = (double)P[i].y + gamma * (double)H[i].y;
+    H[i] = d;
+    V c = C0[i];
+    c.x = (double)c.x + step * (double)d.x;
+    c.y = (double)c.y + step * (double)d.y;
+    C0[i] = c;
+  }
+}
+
+// upper-triangular clean-up after trtri (rocsolver leaves the strict lower
+// triangle untouched) and occupation masking of the Lagrange matrix.
+__global__ void k_triu(double* A, int n, int lda) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n * n; i += gridDim.x * blockDim.x) {
+    const int r = i % n, c = i / n;
+    if (r > c) A[(size_t)c * lda + r] = 0.0;
+  }
+}
+__global__ void k_mask_occ(double* A, int n, const double* __restrict__ occ) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n * n; i += gridDim.x * blockDim.x) {
+    const int r = i % n, c = i / n;
+    if (occ[r] == 0.0 || occ[c] == 0.0) A[i] = 0.0;
+  }
+}
+
+}  // namespace amdk

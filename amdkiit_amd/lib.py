@@ -33,6 +33,24 @@ class Desc(C.Structure):
     ]
 
 
+class Species(C.Structure):
+    _fields_ = [
+        ("nsp", C.c_int), ("maxproj", C.c_int),
+        ("zv", _dp), ("vps", _dp), ("rhos", _dp),
+        ("nproj", _ip), ("proj_l", _ip), ("twnl", _dp), ("dmat", _dp),
+    ]
+
+
+class Energies(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("etot", "ekin", "estat", "eself", "esr", "eloc", "enl", "exc", "eh", "gemax", "cnorm", "nelec")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+XC = {"TETER93": 0, "PZ": 1, "PBE": 2}
+
 SIGNATURES = {
     "amdk_ctx_create": (_i, [C.POINTER(Desc), C.POINTER(_vp)]),
     "amdk_ctx_destroy": (None, [_vp]),
@@ -44,9 +62,25 @@ SIGNATURES = {
     "amdk_vpsi_dev": (_i, [_vp, _vp, _vp, _dp, _i]),
     "amdk_density": (_i, [_vp, _dp, _i, _dp, _i, _dp]),
     "amdk_density_dev": (_i, [_vp, _vp, _dp, _i, _vp]),
+    "amdk_set_species": (_i, [_vp, C.POINTER(Species)]),
+    "amdk_set_positions": (_i, [_vp, _i, _ip, _dp]),
+    "amdk_nproj": (_i, [_vp]),
+    "amdk_fnl": (_i, [_vp, _dp, _i, _i, _dp]),
+    "amdk_fnonloc": (_i, [_vp, _dp, _dp, _i, _dp, _i]),
+    "amdk_overlap": (_i, [_vp, _dp, _dp, _i, _i, _i, _dp]),
+    "amdk_lagrange": (_i, [_vp, _dp, _dp, _i, _dp, _i, _dp, _dp]),
+    "amdk_ortho": (_i, [_vp, _dp, _i, _i]),
+    "amdk_vks": (_i, [_vp, _dp, _i, _dp, C.POINTER(Energies)]),
+    "amdk_forces": (_i, [_vp, _dp, _i, _dp, _i, _dp]),
+    "amdk_wfn_load": (_i, [_vp, _dp, _i, _dp, _i]),
+    "amdk_wfn_store": (_i, [_vp, _dp, _i]),
+    "amdk_scf_step": (_i, [_vp, _i, _i, C.POINTER(Energies)]),
+    "amdk_scf_forces": (_i, [_vp, _dp]),
     "amdk_bands_to_dev": (_i, [_vp, _dp, _i, _i, _vp]),
     "amdk_bands_from_dev": (_i, [_vp, _vp, _i, _dp, _i]),
+    "amdk_sphere_info": (_i, [_vp, _i, _ip]),
     "amdk_profile_enable": (_i, [_vp, _i]),
+    "amdk_profile_names": (_i, [_vp, C.c_char_p, _i]),
     "amdk_profile_query": (_i, [_vp, C.c_char_p, _dp, _ip]),
     "amdk_profile_reset": (_i, [_vp]),
 }

@@ -1,0 +1,216 @@
+"""Benchmark: Kohn-Sham SCF steps/s and H|psi> GB/s on one MI355X.
+
+Workload (default): 32 H2O at 1.0 g/cm^3 in an 18.63-bohr cubic box, GTH-PBE,
+70 Ry (100^3 grid, 31,995 half-sphere plane waves), 128 doubly occupied
+bands; mixed precision (FP32 FFT/H|psi>, FP64 overlap accumulation, Cholesky,
+energies).  One "step" is one iteration of the reference's wfn_opt loop
+(density -> V_KS -> energies -> H|psi> -> Lagrange -> PCG -> ortho) on
+device-resident wavefunctions.  Synthetic data: seeded water lattice and
+seeded random initial coefficients.
+
+Prints one JSON line (see DESIGN.md for the fields).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SCF steps/sec + H|ψ⟩ GB/s on 32/64 H2O; mixed vs FP64 GPU at 1e-6 Ha/atom"
+HBM_PEAK = 8000.0  # GB/s, MI355X HBM3E
+HPSI_KERNELS = ("rows_inv", "cols_inv", "z_vpsi", "cols_fwd", "rows_fwd_hpsi")
+CONFIGS = {
+    "h2o_32_pbe": "32 H2O PBE 70 Ry, 18.63 bohr, 128 bands, 100^3",
+    "h2o_64_pbe": "64 H2O PBE 80 Ry, 23.47 bohr, 256 bands, 144^3",
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def hpsi_bytes(pw, nb, prec):
+    """Algorithmic bytes of one local H|psi> application (all bands):
+    (nb/2) * (36 N_r + 48 N_pw) in FP32, doubled in FP64."""
+    s = 1 if prec == "mixed" else 2
+    return (nb / 2.0) * (36.0 * pw.nr + 48.0 * pw.ngpw) * s
+
+
+def kernel_bytes(ctx, pw, nb, prec):
+    """Algorithmic bytes per launch of each H|psi> pass (all pairs in one launch):
+    the sphere coefficients are FP64 (16 B), the row/plane intermediates are in
+    the FFT precision."""
+    cb = 8 if prec == "mixed" else 16
+    rb = cb // 2
+    n1, n2, n3 = pw.n
+    geo = ctx.sphere_geometry()
+    npair = (nb + 1) // 2
+    pg = 4 if prec == "mixed" else 2
+    plane = geo["nplane"] * n1 * n2 * cb
+    rows = geo["nslot"] * n1 * cb
+    return {
+        "rows_inv": npair * (2 * pw.ngpw * 16 + rows),
+        "cols_inv": npair * (rows + plane),
+        "z_vpsi": npair * (2 * plane) + ((npair + pg - 1) // pg) * pw.nr * rb,
+        "cols_fwd": npair * (plane + rows),
+        "rows_fwd_hpsi": npair * (rows + 4 * pw.ngpw * 16),
+    }
+
+
+def run_gpu(sysobj, prec, steps, warmup, c0, profile_steps=3):
+    import torch
+
+    from amdkiit_amd.context import KSContext
+
+    s = sysobj
+    ctx = KSContext(s.pw, s.nbands, prec)
+    ctx.set_species(s.tables)
+    ctx.set_positions(s.sp, s.tau)
+    c = ctx.ortho(c0)
+    ctx.wfn_load(c, s.occ)
+    it = 0
+    for _ in range(warmup):
+        it += 1
+        ctx.scf_step(it, s.xc)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e = None
+    for _ in range(steps):
+        it += 1
+        e = ctx.scf_step(it, s.xc)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    # per-kernel HIP-event timing on the library stream, separate pass
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(profile_steps):
+        it += 1
+        ctx.scf_step(it, s.xc)
+    ctx.profile(False)
+    prof = {}
+    for name in ctx.profile_names():
+        tot, cnt = ctx.profile_query(name)
+        prof[name] = (tot, cnt)
+    return ctx, dt, e, prof, it
+
+
+def cpu_baseline(s, c0, seconds_hint):
+    """FP64 numpy/scipy restatement of the same SCF step on the host cores."""
+    from oracle.ks_oracle import KSOracle
+
+    ncores = os.cpu_count() or 1
+    orc = KSOracle(s.pw, s.tables, s.sp, s.tau, s.xc, workers=ncores)
+    c = c0.copy()
+    t0 = time.perf_counter()
+    c, _ = orc.scf_step(c, s.occ, 1)  # warm-up (plans, caches)
+    t1 = time.perf_counter()
+    n = 0
+    while True:
+        c, _ = orc.scf_step(c, s.occ, 2 + n)
+        n += 1
+        if time.perf_counter() - t1 > seconds_hint or n >= 3:
+            break
+    dt = (time.perf_counter() - t1) / n
+    return {"value": 1.0 / dt, "unit": "SCF steps/s", "cores": ncores, "kind": "port",
+            "sample": f"{n} SCF step(s) of the same workload after 1 warm-up step, FP64 numpy/scipy "
+                      f"(scipy.fft workers={ncores}, threaded BLAS)",
+            "s_per_step": dt, "warmup_s": t1 - t0}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="h2o_32_pbe", choices=sorted(CONFIGS))
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
+              "the multi-GPU band sharding is a design in DESIGN.md, not built.", file=sys.stderr)
+        sys.exit(2)
+
+    from amdkiit_amd import systems
+    from amdkiit_amd.pwgrid import PWSetup  # noqa: F401
+
+    log(f"[bench] building {a.config}")
+    s = systems.CONFIGS[a.config]()
+    rng = np.random.default_rng(2024)
+    g2 = s.pw.g2_pw
+    c0 = (rng.random((s.nbands, s.pw.ngpw)) - 0.5 + 1j * (rng.random((s.nbands, s.pw.ngpw)) - 0.5)) / (1.0 + g2)
+    c0[:, 0] = c0[:, 0].real
+    log(f"[bench] n={s.pw.n} ngpw={s.pw.ngpw} nb={s.nbands} nat={s.nat}")
+
+    ctx, dt, e, prof, it = run_gpu(s, a.precision, a.steps, a.warmup, c0)
+    ms = dt / a.steps * 1e3
+    value = a.steps / dt
+    log(f"[bench] {a.precision}: {ms:.3f} ms/step  E={e['etot']:.8f}  gemax={e['gemax']:.3e}")
+
+    kb = kernel_bytes(ctx, s.pw, s.nbands, a.precision)
+    hp_ms = sum(prof[k][0] / prof[k][1] for k in HPSI_KERNELS if k in prof and prof[k][1])
+    hb = hpsi_bytes(s.pw, s.nbands, a.precision)
+    hp_gbs = hb / (hp_ms * 1e-3) / 1e9
+    per_kernel = {}
+    for k in HPSI_KERNELS:
+        if k in prof and prof[k][1]:
+            avg = prof[k][0] / prof[k][1]
+            per_kernel[k] = {"avg_ms": avg, "alg_bytes": kb[k], "GBps": kb[k] / (avg * 1e-3) / 1e9}
+    step_breakdown = {k: v[0] / 3 for k, v in sorted(prof.items(), key=lambda x: -x[1][0])}
+    dominant = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"])
+
+    fp64 = None
+    if not a.no_fp64 and a.precision == "mixed":
+        ctx.close()
+        steps64 = max(3, a.steps // 2)
+        ctx64, dt64, e64, prof64, _ = run_gpu(s, "fp64", steps64, a.warmup, c0, profile_steps=1)
+        fp64 = {"steps_per_s": steps64 / dt64, "ms_per_step": dt64 / steps64 * 1e3,
+                "mixed_speedup": value / (steps64 / dt64)}
+        ctx64.close()
+        log(f"[bench] fp64: {fp64['ms_per_step']:.3f} ms/step; mixed speedup {fp64['mixed_speedup']:.2f}x")
+    else:
+        ctx.close()
+
+    cpu = None
+    if not a.no_cpu:
+        log("[bench] cpu baseline (FP64 port) ...")
+        cpu = cpu_baseline(s, c0, a.cpu_seconds)
+        log(f"[bench] cpu: {cpu['s_per_step']:.2f} s/step on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "SCF steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if a.precision == "mixed" else "f64",
+        "data": "synthetic (seeded water lattice + seeded random coefficients)",
+        "config": {"workload": CONFIGS[a.config], "precision": a.precision, "grid": list(s.pw.n),
+                   "ngpw": s.pw.ngpw, "bands": s.nbands, "atoms": s.nat, "xc": s.xc},
+        "hpsi": {"ms": hp_ms, "alg_bytes": hb, "GBps": hp_gbs, "frac_hbm": hp_gbs / HBM_PEAK,
+                 "launches": list(HPSI_KERNELS)},
+        "roofline": {"bound": "hbm", "kernel": "hpsi_local (" + "+".join(HPSI_KERNELS) + ")",
+                     "achieved": hp_gbs, "peak": HBM_PEAK, "unit": "GB/s", "frac": hp_gbs / HBM_PEAK,
+                     "traffic": None, "dominant_launch": dominant, "per_launch": per_kernel},
+        "step_ms_by_region": step_breakdown,
+        "energy": {"etot": e["etot"], "gemax": e["gemax"], "nelec": e["nelec"], "after_steps": it},
+        "fp64_gpu": fp64,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

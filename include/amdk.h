@@ -21,8 +21,11 @@
  *    on return.  One host thread per context.
  *  - *_dev entry points take device pointers in the context's internal
  *    layout (plane-wave order of amdk_pw_order, leading dimension
- *    amdk_ldc(), element type float2 in AMDK_PREC_MIXED, double2 in
- *    AMDK_PREC_FP64) so that wavefunctions stay resident in HBM.
+ *    amdk_ldc(), element type double2) so that wavefunctions stay resident
+ *    in HBM.  Coefficients are stored in FP64 in both precision modes; the
+ *    mode selects the arithmetic of the FFTs and of V(r) psi(r)
+ *    (AMDK_PREC_MIXED: FP32) while overlaps, Cholesky, energies and the
+ *    density-grid potential stay FP64.
  */
 #ifndef AMDK_H
 #define AMDK_H
@@ -81,14 +84,93 @@ int amdk_vpsi_dev(amdk_ctx* ctx, const void* c0, void* c2, const double* occ, in
 int amdk_density(amdk_ctx* ctx, const double* c0, int ldc, const double* occ, int nb, double* rho_r);
 int amdk_density_dev(amdk_ctx* ctx, const void* c0, const double* occ, int nb, double* rho_r_dev);
 
+/* Species tables, all on the caller's sphere orders (set once per run).
+ * vps/rhos: V_s(G)/Omega and -Z exp(-G^2 r_c^2/4)/Omega on the density
+ * sphere (form_factor, exp_igr.f90:142-288); twnl: Y_lm(G^) w(|G|)/sqrt(Omega)
+ * on the wavefunction sphere (nlpro, pseudopotential.f90:108-162, computed
+ * once instead of every step); dmat: projector coupling per species (the
+ * diagonal KB weights wsg of nlppgrid, pseudopotential.f90:434-529, or UPF
+ * hlsg blocks).  Replaces the module globals vps, rhos, twnl, wsg/hlsg. */
+typedef struct {
+  int nsp, maxproj;
+  const double* zv;      /* [nsp] valence charge */
+  const double* vps;     /* [nsp][ngrho] */
+  const double* rhos;    /* [nsp][ngrho] */
+  const int* nproj;      /* [nsp] projectors of the species (ngh) */
+  const int* proj_l;     /* [nsp][maxproj] angular momentum (nghtol) */
+  const double* twnl;    /* [nsp][maxproj][ngpw] */
+  const double* dmat;    /* [nsp][maxproj][maxproj] */
+} amdk_species;
+int amdk_set_species(amdk_ctx* ctx, const amdk_species* s);
+
+/* Atomic positions (bohr, tau[3*a+k]) and species of each atom.  Builds the
+ * structure factors, sum_a V_s e^{-iGR}, sum_a rho_s e^{-iGR}, the projector
+ * table beta = (-i)^l twnl e^{-iGR} and the short-range ion energy/forces.
+ * Replaces exp_igrxfactor (exp_igr.f90:4-140) and E_SR (total_energy.f90:133-239). */
+int amdk_set_positions(amdk_ctx* ctx, int nat, const int* sp_of_atom, const double* tau);
+/* Number of projector rows (sum over atoms of nproj) = rows of nl. */
+int amdk_nproj(const amdk_ctx* ctx);
+
+/* nl(p, b) = <beta_p|psi_b> (full sphere), column-major (nproj, nb).
+ * Replaces fnl(C0, nstat) (pseudopotential.f90:164-226). */
+int amdk_fnl(amdk_ctx* ctx, const double* c0, int ldc, int nb, double* nl);
+/* c2 += -f~ sum_p beta_p (D nl)_p with f~ = f (>= 1e-5) else 1.
+ * Replaces FNONLOC(CA, NOSTATE, FC) (wfn_initialize.f90:560-612). */
+int amdk_fnonloc(amdk_ctx* ctx, const double* nl, double* c2, int ldc, const double* occ, int nb);
+/* A(i,j) = 2 Re <ca_i|cb_j>_half - ca_i(0) cb_j(0), column-major (n1, n2).
+ * Replaces OVLAP_D(NG_PW, N1, N2, A, CA, CB) (wfn_initialize.f90:234-263). */
+int amdk_overlap(amdk_ctx* ctx, const double* ca, const double* cb, int ldc, int n1, int n2, double* A);
+/* Gradient projection: G = <c2|c0> masked by occupation, c2 -= c0 G,
+ * Im c2(G=0) = 0; returns GEMAX and CNORM.
+ * Replaces FORCES + eval_GMAX (gradient.f90:5-61). */
+int amdk_lagrange(amdk_ctx* ctx, const double* c0, double* c2, int ldc, const double* occ, int nb,
+                  double* gemax, double* cnorm);
+/* Cholesky orthonormalisation in FP64: S = U^T U, c <- c U^-1, Im c(0) = 0.
+ * Returns AMDK_ERR_NOT_PD where UINV would STOP.
+ * Replaces ortho_gs(CP) (wfn_initialize.f90:448-558). */
+int amdk_ortho(amdk_ctx* ctx, double* c, int ldc, int nb);
+
+enum { AMDK_XC_TETER93 = 0, AMDK_XC_PZ = 1, AMDK_XC_PBE = 2 };
+
+/* Energies of one SCF step (Hartree), as printed by eprint.f90. */
+typedef struct {
+  double etot, ekin, estat, eself, esr, eloc, enl, exc, eh;
+  double gemax, cnorm, nelec;
+} amdk_energies;
+
+/* From rho(r): rho(G) (rhor2g, density.f90:128-150), V_H + V_loc
+ * (eval_pot, potential.f90:4-46) and V_xc (cal_vpot_ex, xc.f90:4-497) ->
+ * V_KS(r) kept on the device (and copied to v_r when non-null).
+ * e receives eh, eloc, exc (others untouched). */
+int amdk_vks(amdk_ctx* ctx, const double* rho_r, int xc, double* v_r, amdk_energies* e);
+
+/* Forces -dE/dR (Ha/bohr) [3*nat]: POTFOR + NLFOR/D_FNL + E_SR
+ * (nuclear_grad.f90:4-372), using rho(G) of the last amdk_vks call. */
+int amdk_forces(amdk_ctx* ctx, const double* c0, int ldc, const double* occ, int nb, double* f);
+
+/* Device-resident SCF (wfn_opt loop body, do_opt.f90:80-129): load C_0,
+ * run steps, read C_0 back.  istep is 1-based within one optimisation (it
+ * drives the PCG momentum schedule of cg.f90:88-129). */
+int amdk_wfn_load(amdk_ctx* ctx, const double* c0, int ldc, const double* occ, int nb);
+int amdk_wfn_store(amdk_ctx* ctx, double* c0, int ldc);
+int amdk_scf_step(amdk_ctx* ctx, int istep, int xc, amdk_energies* e);
+/* Forces for the resident C_0 and the rho(G) of the last SCF step. */
+int amdk_scf_forces(amdk_ctx* ctx, double* f);
+
 /* Transfer helpers between caller-layout FP64 host arrays and the internal
  * device layout (no reference counterpart: they replace nothing, they keep
  * C_0/C2 resident across an SCF step). */
 int amdk_bands_to_dev(amdk_ctx* ctx, const double* c, int ldc, int nb, void* dst);
 int amdk_bands_from_dev(amdk_ctx* ctx, const void* src, int nb, double* c, int ldc);
 
+/* Sphere geometry as the FFT kernels see it: which = 0 wavefunction sphere,
+ * 1 density sphere; out = {npw, ldc, half rows, full rows, planes, R}. */
+int amdk_sphere_info(const amdk_ctx* ctx, int which, int* out6);
+
 /* Per-kernel timing with HIP events on the context stream. */
 int amdk_profile_enable(amdk_ctx* ctx, int on);
+/* Names of the timed regions so far, '\n'-separated, into buf[len]. */
+int amdk_profile_names(amdk_ctx* ctx, char* buf, int len);
 int amdk_profile_query(amdk_ctx* ctx, const char* kernel, double* total_ms, int* count);
 int amdk_profile_reset(amdk_ctx* ctx);
 

@@ -1,0 +1,178 @@
+"""GPU parity of the SCF-step kernels against the FP64 oracle, and full SCF
+runs against the reference's recorded energies."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from amdkiit_amd import systems
+from amdkiit_amd.context import KSContext
+from oracle.ks_oracle import KSOracle
+from tests.common import random_bands
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF = json.load(open(os.path.join(HERE, "golden", "reference_energies.json")))
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def build(sysf, prec):
+    s = sysf()
+    ctx = KSContext(s.pw, s.nbands, prec)
+    ctx.set_species(s.tables)
+    ctx.set_positions(s.sp, s.tau)
+    orc = KSOracle(s.pw, s.tables, s.sp, s.tau, s.xc, workers=8)
+    return s, ctx, orc
+
+
+SMALL = {
+    "h2o2_pbe": lambda: systems.water_box((1, 1, 2), 10.0, 20.0, "PBE"),
+    "h2o2_pz": lambda: systems.water_box((1, 1, 2), 10.0, 20.0, "PZ"),
+    "si8_lda": lambda: systems.si64_like(cells=1, ecut=12.0),
+}
+TOL = {"fp64": 1e-11, "mixed": 3e-5}
+
+
+@pytest.fixture(scope="module", params=list(SMALL), ids=list(SMALL))
+def sysname(request):
+    return request.param
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_projector_and_overlap_gemms(sysname, prec):
+    s, ctx, orc = build(SMALL[sysname], prec)
+    print(f"case={sysname} prec={prec} n={s.pw.n} nb={s.nbands} nproj={ctx.nproj}", flush=True)
+    c = orc.ortho(random_bands(s.pw, s.nbands, seed=3))
+    occ = s.occ
+    nl = ctx.fnl(c)
+    nl_ref = orc.fnl(c)
+    if len(nl_ref):
+        assert rel(nl, nl_ref) < TOL[prec]
+    c2 = random_bands(s.pw, s.nbands, seed=4)
+    got = ctx.fnonloc(nl_ref, c2, occ)
+    ref = orc.fnonloc(c2, nl_ref, occ)
+    assert rel(got, ref) < TOL[prec]
+    A = ctx.overlap(c, c2)
+    assert rel(A, orc.ovlap(c, c2)) < TOL[prec]
+    ctx.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_lagrange_and_ortho(sysname, prec):
+    s, ctx, orc = build(SMALL[sysname], prec)
+    print(f"case={sysname} prec={prec}", flush=True)
+    c = orc.ortho(random_bands(s.pw, s.nbands, seed=5))
+    c2 = random_bands(s.pw, s.nbands, seed=6)
+    occ = s.occ.copy()
+    occ[-1] = 0.0
+    got, gm, cn = ctx.lagrange(c, c2, occ)
+    ref, gm_r, cn_r = orc.lagrange(c, c2, occ)
+    assert rel(got, ref) < TOL[prec]
+    assert abs(gm - gm_r) < 10 * TOL[prec] * gm_r
+    assert abs(cn - cn_r) < 10 * TOL[prec] * cn_r
+    raw = random_bands(s.pw, s.nbands, seed=7)
+    o = ctx.ortho(raw)
+    o_ref = orc.ortho(raw)
+    assert rel(o, o_ref) < (1e-10 if prec == "fp64" else 1e-5)
+    S = orc.ovlap(o, o)
+    assert np.abs(S - np.eye(len(S))).max() < (1e-12 if prec == "fp64" else 1e-6)
+    assert np.all(o[:, 0].imag == 0.0)
+    ctx.close()
+
+
+def test_ortho_rejects_dependent_bands():
+    s, ctx, orc = build(SMALL["h2o2_pz"], "fp64")
+    c = random_bands(s.pw, s.nbands, seed=8)
+    c[1] = 0.0  # a null band: the second pivot of S is exactly zero
+    from amdkiit_amd.lib import AmdkError, ERR_NOT_PD
+    with pytest.raises(AmdkError) as ei:
+        ctx.ortho(c)
+    assert ei.value.code == ERR_NOT_PD
+    ctx.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_vks_and_energies(sysname, prec):
+    s, ctx, orc = build(SMALL[sysname], prec)
+    print(f"case={sysname} prec={prec} xc={s.xc}", flush=True)
+    c = orc.ortho(random_bands(s.pw, s.nbands, seed=9))
+    rho = orc.density(c, s.occ)
+    v, e = ctx.vks(rho, s.xc)
+    rho_g = orc.rhor2g(rho)
+    v_ref, exc_ref = orc.vks(rho, rho_g)
+    # the dense path is FP64 in both precision modes
+    assert rel(v, v_ref) < 1e-10
+    assert abs(e["exc"] - exc_ref) < 1e-10 * abs(exc_ref)
+    nl = orc.fnl(c)
+    er = orc.energies(c, s.occ, rho_g, nl, exc_ref)
+    assert abs(e["eh"] - er["eh"]) < 1e-10 * abs(er["eh"])
+    assert abs(e["eloc"] - er["eloc"]) < 1e-10 * abs(er["eloc"])
+    assert abs(e["nelec"] - s.occ.sum()) < 1e-10 * s.occ.sum()
+    ctx.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_scf_steps_match_oracle(sysname, prec):
+    s, ctx, orc = build(SMALL[sysname], prec)
+    print(f"case={sysname} prec={prec}", flush=True)
+    c = orc.ortho(random_bands(s.pw, s.nbands, seed=10))
+    ctx.wfn_load(c, s.occ)
+    c_ref = c.copy()
+    tol_e = 1e-10 if prec == "fp64" else 2e-6
+    for it in range(1, 4):
+        e = ctx.scf_step(it, s.xc)
+        c_ref, er = orc.scf_step(c_ref, s.occ, it)
+        for k in ("etot", "ekin", "eloc", "enl", "exc", "eh"):
+            assert abs(e[k] - er[k]) < tol_e * max(1.0, abs(er[k])), (it, k, e[k], er[k])
+        assert abs(e["gemax"] - er["gemax"]) < 1e3 * tol_e * er["gemax"]
+    got = ctx.wfn_store()
+    assert rel(got, c_ref) < (1e-9 if prec == "fp64" else 1e-4)
+    ctx.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_forces_match_oracle(sysname, prec):
+    s, ctx, orc = build(SMALL[sysname], prec)
+    print(f"case={sysname} prec={prec}", flush=True)
+    c = orc.ortho(random_bands(s.pw, s.nbands, seed=11))
+    rho = orc.density(c, s.occ)
+    ctx.vks(rho, s.xc)
+    f = ctx.forces(c, s.occ)
+    rho_g = orc.rhor2g(rho)
+    f_ref = orc.forces(c, s.occ, rho_g, orc.fnl(c))
+    assert np.abs(f - f_ref).max() < (1e-9 if prec == "fp64" else 1e-5)
+    ctx.close()
+
+
+def run_scf(ctx, s, c0, gcut, maxit=400):
+    ctx.wfn_load(c0, s.occ)
+    for it in range(1, maxit + 1):
+        e = ctx.scf_step(it, s.xc)
+        if e["gemax"] < gcut:
+            return e, it
+    return e, maxit
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("nmol", [1, 32])
+def test_scf_reproduces_recorded_reference_energy(nmol, prec):
+    ref = REF[f"h2o_{nmol}"]
+    s = systems.reference_water(nmol)
+    print(f"case=h2o_{nmol} prec={prec} n={s.pw.n} nb={s.nbands}", flush=True)
+    ctx = KSContext(s.pw, s.nbands, prec)
+    ctx.set_species(s.tables)
+    ctx.set_positions(s.sp, s.tau)
+    orc = KSOracle(s.pw, s.tables, s.sp, s.tau, s.xc)
+    c0 = orc.ortho(random_bands(s.pw, s.nbands, seed=1))
+    e, it = run_scf(ctx, s, c0, 1e-7 if nmol == 1 else 1e-6)
+    print(f"  steps={it} etot={e['etot']:.10f} ref={ref['etot']:.10f} nelec={e['nelec']:.12f}", flush=True)
+    tol = 1e-6 * s.nat  # 1e-6 Ha/atom
+    assert abs(e["etot"] - ref["etot"]) < tol
+    assert abs(e["esr"] - ref["esr"]) < 1e-9
+    assert abs(e["eself"] - ref["eself"]) < 1e-9
+    assert abs(e["nelec"] - s.occ.sum()) < 1e-10 * s.occ.sum()
+    ctx.close()
