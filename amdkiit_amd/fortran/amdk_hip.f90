@@ -1,0 +1,261 @@
+! amdk_hip: ISO_C_BINDING interface to libamdk (include/amdk.h) and drop-in
+! shims with the argument lists of the reference's hot-path subroutines.
+!
+! The reference keeps its call sites; each shim is called from the body of the
+! routine it replaces, passing the module globals the routine used implicitly
+! (NGPW_l, the context handle).  A non-zero status prints amdk_last_error and
+! STOPs, which is what the reference does on LAPACK failures
+! (wfn_initialize.f90:513-527).
+module amdk_hip
+  use iso_c_binding
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: AMDK_OK = 0, AMDK_ERR_NOT_PD = 4
+  integer(c_int), parameter, public :: AMDK_PREC_FP64 = 0, AMDK_PREC_MIXED = 1
+  integer(c_int), parameter, public :: AMDK_XC_TETER93 = 0, AMDK_XC_PZ = 1, AMDK_XC_PBE = 2
+
+  type, bind(C), public :: amdk_desc
+    integer(c_int) :: n1, n2, n3
+    real(c_double) :: a_cell(9)
+    integer(c_int) :: ngpw
+    type(c_ptr) :: miller_pw
+    integer(c_int) :: ngrho
+    type(c_ptr) :: miller_rho
+    integer(c_int) :: nbmax, precision
+  end type
+
+  type, bind(C), public :: amdk_species
+    integer(c_int) :: nsp, maxproj
+    type(c_ptr) :: zv, vps, rhos, nproj, proj_l, twnl, dmat
+  end type
+
+  type, bind(C), public :: amdk_energies
+    real(c_double) :: etot, ekin, estat, eself, esr, eloc, enl, exc, eh
+    real(c_double) :: gemax, cnorm, nelec
+  end type
+
+  public :: amdk_ctx_create, amdk_ctx_destroy, amdk_set_vks, amdk_vpsi, amdk_density, &
+            amdk_set_species, amdk_set_positions, amdk_nproj, amdk_fnl, amdk_fnonloc, &
+            amdk_overlap, amdk_lagrange, amdk_ortho, amdk_vks, amdk_forces, &
+            amdk_wfn_load, amdk_wfn_store, amdk_scf_step, amdk_scf_forces
+  public :: amdk_check, vpsi_hip, fnonloc_hip, fnl_hip, ovlap_d_hip, forces_hip, ortho_gs_hip, &
+            eval_density_hip
+
+  interface
+    integer(c_int) function amdk_ctx_create(d, ctx) bind(C, name="amdk_ctx_create")
+      import :: c_int, c_ptr, amdk_desc
+      type(amdk_desc), intent(in) :: d
+      type(c_ptr), intent(out) :: ctx
+    end function
+    subroutine amdk_ctx_destroy(ctx) bind(C, name="amdk_ctx_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end subroutine
+    type(c_ptr) function amdk_last_error(ctx) bind(C, name="amdk_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function amdk_set_vks(ctx, v) bind(C, name="amdk_set_vks")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: v(*)
+    end function
+    integer(c_int) function amdk_vpsi(ctx, c0, ldc, c2, occ, nb) bind(C, name="amdk_vpsi")
+      import :: c_int, c_ptr, c_double_complex, c_double
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: c0(*)
+      integer(c_int), value :: ldc, nb
+      complex(c_double_complex), intent(inout) :: c2(*)
+      real(c_double), intent(in) :: occ(*)
+    end function
+    integer(c_int) function amdk_density(ctx, c0, ldc, occ, nb, rho) bind(C, name="amdk_density")
+      import :: c_int, c_ptr, c_double_complex, c_double
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: c0(*)
+      integer(c_int), value :: ldc, nb
+      real(c_double), intent(in) :: occ(*)
+      real(c_double), intent(out) :: rho(*)
+    end function
+    integer(c_int) function amdk_set_species(ctx, s) bind(C, name="amdk_set_species")
+      import :: c_int, c_ptr, amdk_species
+      type(c_ptr), value :: ctx
+      type(amdk_species), intent(in) :: s
+    end function
+    integer(c_int) function amdk_set_positions(ctx, nat, sp, tau) bind(C, name="amdk_set_positions")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nat
+      integer(c_int), intent(in) :: sp(*)
+      real(c_double), intent(in) :: tau(*)
+    end function
+    integer(c_int) function amdk_nproj(ctx) bind(C, name="amdk_nproj")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function amdk_fnl(ctx, c0, ldc, nb, nl) bind(C, name="amdk_fnl")
+      import :: c_int, c_ptr, c_double_complex, c_double
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: c0(*)
+      integer(c_int), value :: ldc, nb
+      real(c_double), intent(out) :: nl(*)
+    end function
+    integer(c_int) function amdk_fnonloc(ctx, nl, c2, ldc, occ, nb) bind(C, name="amdk_fnonloc")
+      import :: c_int, c_ptr, c_double_complex, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: nl(*)
+      complex(c_double_complex), intent(inout) :: c2(*)
+      integer(c_int), value :: ldc, nb
+      real(c_double), intent(in) :: occ(*)
+    end function
+    integer(c_int) function amdk_overlap(ctx, ca, cb, ldc, n1, n2, a) bind(C, name="amdk_overlap")
+      import :: c_int, c_ptr, c_double_complex, c_double
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: ca(*), cb(*)
+      integer(c_int), value :: ldc, n1, n2
+      real(c_double), intent(out) :: a(*)
+    end function
+    integer(c_int) function amdk_lagrange(ctx, c0, c2, ldc, occ, nb, gemax, cnorm) bind(C, name="amdk_lagrange")
+      import :: c_int, c_ptr, c_double_complex, c_double
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: c0(*)
+      complex(c_double_complex), intent(inout) :: c2(*)
+      integer(c_int), value :: ldc, nb
+      real(c_double), intent(in) :: occ(*)
+      real(c_double), intent(out) :: gemax, cnorm
+    end function
+    integer(c_int) function amdk_ortho(ctx, c, ldc, nb) bind(C, name="amdk_ortho")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(inout) :: c(*)
+      integer(c_int), value :: ldc, nb
+    end function
+    integer(c_int) function amdk_vks(ctx, rho, xc, v, e) bind(C, name="amdk_vks")
+      import :: c_int, c_ptr, c_double, amdk_energies
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: rho(*)
+      integer(c_int), value :: xc
+      real(c_double), intent(out) :: v(*)
+      type(amdk_energies), intent(inout) :: e
+    end function
+    integer(c_int) function amdk_forces(ctx, c0, ldc, occ, nb, f) bind(C, name="amdk_forces")
+      import :: c_int, c_ptr, c_double_complex, c_double
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: c0(*)
+      integer(c_int), value :: ldc, nb
+      real(c_double), intent(in) :: occ(*)
+      real(c_double), intent(out) :: f(*)
+    end function
+    integer(c_int) function amdk_wfn_load(ctx, c0, ldc, occ, nb) bind(C, name="amdk_wfn_load")
+      import :: c_int, c_ptr, c_double_complex, c_double
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: c0(*)
+      integer(c_int), value :: ldc, nb
+      real(c_double), intent(in) :: occ(*)
+    end function
+    integer(c_int) function amdk_wfn_store(ctx, c0, ldc) bind(C, name="amdk_wfn_store")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(out) :: c0(*)
+      integer(c_int), value :: ldc
+    end function
+    integer(c_int) function amdk_scf_step(ctx, istep, xc, e) bind(C, name="amdk_scf_step")
+      import :: c_int, c_ptr, amdk_energies
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: istep, xc
+      type(amdk_energies), intent(out) :: e
+    end function
+    integer(c_int) function amdk_scf_forces(ctx, f) bind(C, name="amdk_scf_forces")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: f(*)
+    end function
+    integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
+      import :: c_size_t, c_ptr
+      type(c_ptr), value :: s
+    end function
+  end interface
+
+contains
+
+  ! Non-zero status -> message and STOP (UINV behaviour of the reference).
+  subroutine amdk_check(ctx, rc, where)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    character(kind=c_char), pointer :: msg(:)
+    integer :: n
+    if (rc == AMDK_OK) return
+    n = int(c_strlen(amdk_last_error(ctx)))
+    call c_f_pointer(amdk_last_error(ctx), msg, [n])
+    write (*, '(A,A,I3,A,*(A))') where, ': amdk error ', rc, ' : ', msg(1:n)
+    stop 1
+  end subroutine
+
+  ! VPSI(C_ATOM, C2_nl, FC, NOSTATE)       wfn_initialize.f90:268-365
+  subroutine vpsi_hip(ctx, ngpw, c_atom, c2_nl, fc, nostate)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: ngpw, nostate
+    complex(c_double_complex), intent(in) :: c_atom(ngpw, nostate)
+    complex(c_double_complex), intent(inout) :: c2_nl(ngpw, nostate)
+    real(c_double), intent(in) :: fc(nostate)
+    call amdk_check(ctx, amdk_vpsi(ctx, c_atom, int(ngpw, c_int), c2_nl, fc, int(nostate, c_int)), 'VPSI')
+  end subroutine
+
+  ! FNONLOC(CA, NOSTATE, FC) with the nl of the last fnl   wfn_initialize.f90:560-612
+  subroutine fnonloc_hip(ctx, ngpw, ca, nostate, fc, nl)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: ngpw, nostate
+    complex(c_double_complex), intent(inout) :: ca(ngpw, nostate)
+    real(c_double), intent(in) :: fc(nostate), nl(*)
+    call amdk_check(ctx, amdk_fnonloc(ctx, nl, ca, int(ngpw, c_int), fc, int(nostate, c_int)), 'FNONLOC')
+  end subroutine
+
+  ! fnl(C0, nstat) -> nl(nproj, nstat)        pseudopotential.f90:164-226
+  subroutine fnl_hip(ctx, ngpw, c0, nstat, nl)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: ngpw, nstat
+    complex(c_double_complex), intent(in) :: c0(ngpw, nstat)
+    real(c_double), intent(out) :: nl(*)
+    call amdk_check(ctx, amdk_fnl(ctx, c0, int(ngpw, c_int), int(nstat, c_int), nl), 'fnl')
+  end subroutine
+
+  ! OVLAP_D(NG_PW, N1, N2, A, CA, CB)          wfn_initialize.f90:234-263
+  subroutine ovlap_d_hip(ctx, ng_pw, n1, n2, a, ca, cb)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: ng_pw, n1, n2
+    real(c_double), intent(out) :: a(n1, n2)
+    complex(c_double_complex), intent(in) :: ca(ng_pw, n1), cb(ng_pw, n2)
+    call amdk_check(ctx, amdk_overlap(ctx, ca, cb, int(ng_pw, c_int), int(n1, c_int), int(n2, c_int), a), 'OVLAP_D')
+  end subroutine
+
+  ! FORCES + eval_GMAX                         gradient.f90:5-61
+  subroutine forces_hip(ctx, ngpw, c0, c2, occ, nstate, gemax, cnorm)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: ngpw, nstate
+    complex(c_double_complex), intent(in) :: c0(ngpw, nstate)
+    complex(c_double_complex), intent(inout) :: c2(ngpw, nstate)
+    real(c_double), intent(in) :: occ(nstate)
+    real(c_double), intent(out) :: gemax, cnorm
+    call amdk_check(ctx, amdk_lagrange(ctx, c0, c2, int(ngpw, c_int), occ, int(nstate, c_int), gemax, cnorm), 'FORCES')
+  end subroutine
+
+  ! ortho_gs(CP)                                wfn_initialize.f90:448-558
+  subroutine ortho_gs_hip(ctx, ngpw, cp, nstate)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: ngpw, nstate
+    complex(c_double_complex), intent(inout) :: cp(ngpw, nstate)
+    call amdk_check(ctx, amdk_ortho(ctx, cp, int(ngpw, c_int), int(nstate, c_int)), 'ortho_gs')
+  end subroutine
+
+  ! eval_density                                density.f90:5-98
+  subroutine eval_density_hip(ctx, ngpw, c0, occ, nstate, rho)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: ngpw, nstate
+    complex(c_double_complex), intent(in) :: c0(ngpw, nstate)
+    real(c_double), intent(in) :: occ(nstate)
+    real(c_double), intent(out) :: rho(*)
+    call amdk_check(ctx, amdk_density(ctx, c0, int(ngpw, c_int), occ, int(nstate, c_int), rho), 'eval_density')
+  end subroutine
+
+end module amdk_hip
