@@ -3,11 +3,13 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <tuple>
 
 #include "amdk_ctx.h"
+#include "fft_reg.h"
 
 using namespace amdk;
 
@@ -158,6 +160,11 @@ static void build_sphere(amdk_ctx* c, const int* mill, int ncaller, Sphere& S) {
     if (S.h_rows[r].m2 == 0 && S.h_rows[r].m3 == 0) S.h_rows[r].slot_m = -1;
   for (int p = 0; p < S.nplane; ++p) S.h_plane_off[p + 1] += S.h_plane_off[p];
   if (S.h_plane_off[S.nplane] != S.nslot) throw Error(AMDK_ERR_ARG, "internal: slot count mismatch");
+  // the register passes address a plane's rows as m2_lo + offset
+  S.planes_contiguous = true;
+  for (int p = 0; p < S.nplane; ++p)
+    for (int s = S.h_plane_off[p] + 1; s < S.h_plane_off[p + 1]; ++s)
+      if (S.h_slot_m2[s] != S.h_slot_m2[s - 1] + 1) S.planes_contiguous = false;
   // upload
   S.rows = (RowInfo*)dev_alloc(c, sizeof(RowInfo) * S.nrow);
   S.slot_m2 = (int*)dev_alloc(c, sizeof(int) * S.nslot);
@@ -247,6 +254,79 @@ template <typename T> struct Tiles;
 template <> struct Tiles<float> { static constexpr int RPB = 8, TX = 16, PG = 4; };
 template <> struct Tiles<double> { static constexpr int RPB = 4, TX = 8, PG = 2; };
 
+// ---------------------------------------------------------------------------
+// register-FFT dispatch: lengths with an (A, B) split get the kernels of
+// fft_reg.h; other lengths use the LDS Stockham passes above.
+#define AMDK_REG_SIZES(X) \
+  X(20, 5, 4) X(24, 6, 4) X(30, 6, 5) X(36, 6, 6) X(48, 8, 6) X(60, 10, 6) X(72, 9, 8) X(100, 10, 10) \
+  X(108, 12, 9) X(144, 12, 12) X(180, 15, 12)
+
+static bool reg_ok(const amdk_ctx* c, int n) {
+  if (c->fft_generic) return false;
+  switch (n) {
+#define AMDK_C(N, A, B) case N: return true;
+    AMDK_REG_SIZES(AMDK_C)
+#undef AMDK_C
+    default: return false;
+  }
+}
+
+template <typename T, int TM> struct RegTiles {
+  static constexpr int TX = (sizeof(T) == 4 && TM <= 10) ? 32 : 16;
+  static constexpr int HR = 240 / (2 * TM);
+  static constexpr int PG = sizeof(T) == 4 ? 4 : 2;
+};
+
+template <typename T, int A, int B>
+static void reg_z_vpsi(amdk_ctx* c, const Sphere& S, int np) {
+  using V = typename cplx_of<T>::type;
+  constexpr int N = A * B, TM = MaxAB<A, B>::v, TX = RegTiles<T, TM>::TX, PG = RegTiles<T, TM>::PG;
+  const size_t lds = (size_t)(N + N * TX) * sizeof(V);
+  dim3 grid((c->n1 + TX - 1) / TX, c->n2, (np + PG - 1) / PG);
+  k_z_vpsi_r<T, A, B, TX><<<grid, TX * TM, lds, c->stream>>>(geom_of(c, S), (V*)c->W2, (const T*)c->vr, np, PG,
+                                                            (const V*)(sizeof(T) == 4 ? c->tw_f[2] : c->tw_d[2]));
+}
+template <typename T, int A, int B>
+static void reg_z_rho(amdk_ctx* c, const Sphere& S, int np, double* rho, int accumulate, const double* wpair) {
+  using V = typename cplx_of<T>::type;
+  constexpr int N = A * B, TM = MaxAB<A, B>::v, TX = RegTiles<T, TM>::TX;
+  const size_t lds = (size_t)(N + N * TX) * sizeof(V);
+  dim3 grid((c->n1 + TX - 1) / TX, c->n2);
+  k_z_rho_r<T, A, B, TX><<<grid, TX * TM, lds, c->stream>>>(geom_of(c, S), (const V*)c->W2, rho, np, wpair, accumulate,
+                                                           (const V*)(sizeof(T) == 4 ? c->tw_f[2] : c->tw_d[2]));
+}
+template <typename T, int A, int B>
+static void reg_cols(amdk_ctx* c, const Sphere& S, int np, bool inverse) {
+  using V = typename cplx_of<T>::type;
+  constexpr int N = A * B, TM = MaxAB<A, B>::v, TX = RegTiles<T, TM>::TX;
+  const size_t lds = (size_t)(N + N * TX) * sizeof(V);
+  dim3 grid((c->n1 + TX - 1) / TX, S.nplane, np);
+  const V* tw = (const V*)(sizeof(T) == 4 ? c->tw_f[1] : c->tw_d[1]);
+  if (inverse)
+    k_cols_inv_r<T, A, B, TX><<<grid, TX * TM, lds, c->stream>>>(geom_of(c, S), (const V*)c->W1, (V*)c->W2, tw);
+  else
+    k_cols_fwd_r<T, A, B, TX><<<grid, TX * TM, lds, c->stream>>>(geom_of(c, S), (const V*)c->W2, (V*)c->W1, tw);
+}
+template <typename T, int A, int B>
+static void reg_rows_inv(amdk_ctx* c, const Sphere& S, const void* C, int ldc, int nb, int pair0, int np) {
+  using V = typename cplx_of<T>::type;
+  constexpr int N = A * B, TM = MaxAB<A, B>::v, HR = RegTiles<T, TM>::HR;
+  const size_t lds = (size_t)(N + 2 * HR * (N + 1)) * sizeof(V);
+  dim3 grid((S.nrow + HR - 1) / HR, np);
+  k_rows_inv_r<T, A, B, HR><<<grid, 2 * HR * TM, lds, c->stream>>>(geom_of(c, S), (const double2*)C, ldc, nb, pair0, (V*)c->W1,
+                                                                  (const V*)(sizeof(T) == 4 ? c->tw_f[0] : c->tw_d[0]));
+}
+template <typename T, int A, int B>
+static void reg_rows_fwd_hpsi(amdk_ctx* c, const Sphere& S, const void* C, void* C2, int nb, int pair0, int np) {
+  using V = typename cplx_of<T>::type;
+  constexpr int N = A * B, TM = MaxAB<A, B>::v, HR = RegTiles<T, TM>::HR;
+  const size_t lds = (size_t)(N + 2 * HR * (N + 1)) * sizeof(V);
+  dim3 grid((S.nrow + HR - 1) / HR, np);
+  k_rows_fwd_hpsi_r<T, A, B, HR><<<grid, 2 * HR * TM, lds, c->stream>>>(geom_of(c, S), (const V*)c->W1, (const double2*)C,
+                                                                       (double2*)C2, S.ldc, nb, pair0, c->d_occ, c->rc,
+                                                                       (const V*)(sizeof(T) == 4 ? c->tw_f[0] : c->tw_d[0]));
+}
+
 template <typename T>
 static void launch_rows_inv(amdk_ctx* c, const Sphere& S, const void* C, int ldc, int nb, int pair0, int np) {
   using V = typename cplx_of<T>::type;
@@ -254,6 +334,16 @@ static void launch_rows_inv(amdk_ctx* c, const Sphere& S, const void* C, int ldc
   const size_t lds = (size_t)(c->n1 + 4 * RPB * c->n1) * sizeof(V);
   dim3 grid((S.nrow + RPB - 1) / RPB, np);
   const int ps = prof_begin(c, "rows_inv");
+  if (reg_ok(c, c->n1) && S.planes_contiguous) {
+    switch (c->n1) {
+#define AMDK_C(N, A, B) case N: reg_rows_inv<T, A, B>(c, S, C, ldc, nb, pair0, np); break;
+      AMDK_REG_SIZES(AMDK_C)
+#undef AMDK_C
+    }
+    AMDK_TRY_HIP(hipGetLastError());
+    prof_end(c, ps);
+    return;
+  }
   k_rows_inv<T, RPB><<<grid, 256, lds, c->stream>>>(geom_of(c, S), (const double2*)C, ldc, nb, pair0, (V*)c->W1, c->p1,
                                                      (const V*)(sizeof(T) == 4 ? c->tw_f[0] : c->tw_d[0]));
   AMDK_TRY_HIP(hipGetLastError());
@@ -269,11 +359,31 @@ static void launch_cols(amdk_ctx* c, const Sphere& S, int np, bool inverse) {
   const V* tw = (const V*)(sizeof(T) == 4 ? c->tw_f[1] : c->tw_d[1]);
   if (inverse) {
     const int ps = prof_begin(c, "cols_inv");
+    if (reg_ok(c, c->n2) && S.planes_contiguous) {
+      switch (c->n2) {
+#define AMDK_C(N, A, B) case N: reg_cols<T, A, B>(c, S, np, true); break;
+        AMDK_REG_SIZES(AMDK_C)
+#undef AMDK_C
+      }
+      AMDK_TRY_HIP(hipGetLastError());
+      prof_end(c, ps);
+      return;
+    }
     k_cols_inv<T, TX><<<grid, 256, lds, c->stream>>>(geom_of(c, S), (const V*)c->W1, (V*)c->W2, c->p2, tw);
     AMDK_TRY_HIP(hipGetLastError());
     prof_end(c, ps);
   } else {
     const int ps = prof_begin(c, "cols_fwd");
+    if (reg_ok(c, c->n2) && S.planes_contiguous) {
+      switch (c->n2) {
+#define AMDK_C(N, A, B) case N: reg_cols<T, A, B>(c, S, np, false); break;
+        AMDK_REG_SIZES(AMDK_C)
+#undef AMDK_C
+      }
+      AMDK_TRY_HIP(hipGetLastError());
+      prof_end(c, ps);
+      return;
+    }
     k_cols_fwd<T, TX><<<grid, 256, lds, c->stream>>>(geom_of(c, S), (const V*)c->W2, (V*)c->W1, c->p2, tw);
     AMDK_TRY_HIP(hipGetLastError());
     prof_end(c, ps);
@@ -287,6 +397,16 @@ static void launch_z_vpsi(amdk_ctx* c, const Sphere& S, int np) {
   const size_t lds = (size_t)(c->n3 + 2 * c->n3 * TX) * sizeof(V) + (size_t)c->n3 * TX * sizeof(T);
   dim3 grid((c->n1 + TX - 1) / TX, c->n2, (np + PG - 1) / PG);
   const int ps = prof_begin(c, "z_vpsi");
+  if (reg_ok(c, c->n3)) {
+    switch (c->n3) {
+#define AMDK_C(N, A, B) case N: reg_z_vpsi<T, A, B>(c, S, np); break;
+      AMDK_REG_SIZES(AMDK_C)
+#undef AMDK_C
+    }
+    AMDK_TRY_HIP(hipGetLastError());
+    prof_end(c, ps);
+    return;
+  }
   k_z_vpsi<T, TX><<<grid, 256, lds, c->stream>>>(geom_of(c, S), (V*)c->W2, (const T*)c->vr, np, PG, c->p3,
                                                  (const V*)(sizeof(T) == 4 ? c->tw_f[2] : c->tw_d[2]));
   AMDK_TRY_HIP(hipGetLastError());
@@ -300,6 +420,16 @@ static void launch_rows_fwd_hpsi(amdk_ctx* c, const Sphere& S, const void* C, vo
   const size_t lds = (size_t)(c->n1 + 4 * RPB * c->n1) * sizeof(V);
   dim3 grid((S.nrow + RPB - 1) / RPB, np);
   const int ps = prof_begin(c, "rows_fwd_hpsi");
+  if (reg_ok(c, c->n1) && S.planes_contiguous) {
+    switch (c->n1) {
+#define AMDK_C(N, A, B) case N: reg_rows_fwd_hpsi<T, A, B>(c, S, C, C2, nb, pair0, np); break;
+      AMDK_REG_SIZES(AMDK_C)
+#undef AMDK_C
+    }
+    AMDK_TRY_HIP(hipGetLastError());
+    prof_end(c, ps);
+    return;
+  }
   k_rows_fwd_hpsi<T, RPB><<<grid, 256, lds, c->stream>>>(geom_of(c, S), (const V*)c->W1, (const double2*)C, (double2*)C2, S.ldc, nb,
                                                           pair0, c->d_occ, c->rc, c->p1,
                                                           (const V*)(sizeof(T) == 4 ? c->tw_f[0] : c->tw_d[0]));
@@ -308,13 +438,23 @@ static void launch_rows_fwd_hpsi(amdk_ctx* c, const Sphere& S, const void* C, vo
 }
 
 template <typename T>
-static void launch_z_rho(amdk_ctx* c, const Sphere& S, int np, double* rho, int accumulate) {
+static void launch_z_rho(amdk_ctx* c, const Sphere& S, int np, double* rho, int accumulate, const double* wpair) {
   using V = typename cplx_of<T>::type;
   constexpr int TX = Tiles<T>::TX;
   const size_t lds = (size_t)(c->n3 + 2 * c->n3 * TX) * sizeof(V);
   dim3 grid((c->n1 + TX - 1) / TX, c->n2);
   const int ps = prof_begin(c, "z_rho");
-  k_z_rho<T, TX, 16><<<grid, 256, lds, c->stream>>>(geom_of(c, S), (const V*)c->W2, rho, np, c->d_wpair, accumulate, c->p3,
+  if (reg_ok(c, c->n3)) {
+    switch (c->n3) {
+#define AMDK_C(N, A, B) case N: reg_z_rho<T, A, B>(c, S, np, rho, accumulate, wpair); break;
+      AMDK_REG_SIZES(AMDK_C)
+#undef AMDK_C
+    }
+    AMDK_TRY_HIP(hipGetLastError());
+    prof_end(c, ps);
+    return;
+  }
+  k_z_rho<T, TX, 16><<<grid, 256, lds, c->stream>>>(geom_of(c, S), (const V*)c->W2, rho, np, wpair, accumulate, c->p3,
                                                      (const V*)(sizeof(T) == 4 ? c->tw_f[2] : c->tw_d[2]));
   AMDK_TRY_HIP(hipGetLastError());
   prof_end(c, ps);
@@ -335,23 +475,30 @@ static void run_vpsi(amdk_ctx* c, const void* C0, void* C2, int nb) {
 }
 
 template <typename T>
-static void renormalise_density(amdk_ctx* c, const void* C0, const double* occ, int nb, double* rho);
+static void renormalise_density(amdk_ctx* c, const void* C0, int nb, double* rho);
 
+// Band weights of the density on the device: d_wpair[b] = f_b / Omega (pairs
+// read two consecutive entries), d_wdens[b] = f_b.  Set once per occupation.
+static void prepare_density_weights(amdk_ctx* c, const double* occ, int nb) {
+  std::vector<double> w(nb + 2, 0.0);
+  for (int b = 0; b < nb; ++b) w[b] = occ[b] / c->omega;
+  AMDK_TRY_HIP(hipMemcpyAsync(c->d_wpair, w.data(), sizeof(double) * (nb + 1), hipMemcpyHostToDevice, c->stream));
+  AMDK_TRY_HIP(hipMemcpyAsync(c->d_wdens, occ, sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
+  AMDK_TRY_HIP(hipStreamSynchronize(c->stream));
+}
+
+// rho(r) for bands [0, nb) with the weights of prepare_density_weights
 template <typename T>
-static void run_density(amdk_ctx* c, const void* C0, const double* occ, int nb, double* rho) {
+static void run_density(amdk_ctx* c, const void* C0, int nb, double* rho) {
   const Sphere& S = c->spw;
   const int npairs = (nb + 1) / 2;
-  std::vector<double> w(2 * npairs, 0.0);
-  for (int b = 0; b < nb; ++b) w[b] = occ[b] / c->omega;
   for (int p0 = 0; p0 < npairs; p0 += c->cap_pairs) {
     const int np = std::min(c->cap_pairs, npairs - p0);
-    AMDK_TRY_HIP(hipMemcpyAsync(c->d_wpair, w.data() + 2 * p0, sizeof(double) * 2 * np, hipMemcpyHostToDevice, c->stream));
     launch_rows_inv<T>(c, S, C0, S.ldc, nb, p0, np);
     launch_cols<T>(c, S, np, true);
-    launch_z_rho<T>(c, S, np, rho, p0 > 0);
-    AMDK_TRY_HIP(hipStreamSynchronize(c->stream));  // d_wpair reused by the next chunk
+    launch_z_rho<T>(c, S, np, rho, p0 > 0, c->d_wpair + 2 * p0);
   }
-  if (sizeof(T) == 4) renormalise_density<T>(c, C0, occ, nb, rho);
+  if (sizeof(T) == 4) renormalise_density<T>(c, C0, nb, rho);
 }
 
 // ---------------------------------------------------------------------------
@@ -400,6 +547,10 @@ int amdk_ctx_create(const amdk_desc* d, amdk_ctx** out) {
     ctx->p3 = make_plan(d->n3);
     ctx->prec = d->precision;
     ctx->nbmax = d->nbmax;
+    {
+      const char* f = std::getenv("AMDK_FFT");
+      ctx->fft_generic = f && std::string(f) == "stockham";
+    }
     std::memcpy(ctx->acell, d->a_cell, sizeof(ctx->acell));
     // reciprocal: g_j = 2 pi * row j of inv(A), A columns = a_j
     const double* a = d->a_cell;  // a[3*j+i] = (a_j)_i
@@ -457,6 +608,7 @@ int amdk_ctx_create(const amdk_desc* d, amdk_ctx** out) {
     ctx->W2 = dev_alloc(ctx, w2_pair * ctx->cap_pairs);
     ctx->d_occ = (double*)dev_alloc(ctx, sizeof(double) * (d->nbmax + 1));
     ctx->d_wpair = (double*)dev_alloc(ctx, sizeof(double) * (d->nbmax + 2));
+    ctx->d_wdens = (double*)dev_alloc(ctx, sizeof(double) * (d->nbmax + 1));
     alloc_scf_buffers(ctx);
     AMDK_TRY_HIP(hipStreamSynchronize(ctx->stream));
   } catch (const amdk::Error& e) {
@@ -553,8 +705,9 @@ int amdk_density_dev(amdk_ctx* ctx, const void* c0, const double* occ, int nb, d
   API_BEGIN
   check_nb(ctx, nb);
   if (!c0 || !occ || !rho_r_dev) throw Error(AMDK_ERR_ARG, "null pointer");
-  if (ctx->prec == AMDK_PREC_FP64) run_density<double>(ctx, c0, occ, nb, rho_r_dev);
-  else run_density<float>(ctx, c0, occ, nb, rho_r_dev);
+  prepare_density_weights(ctx, occ, nb);
+  if (ctx->prec == AMDK_PREC_FP64) run_density<double>(ctx, c0, nb, rho_r_dev);
+  else run_density<float>(ctx, c0, nb, rho_r_dev);
   sync(ctx);
   API_END
 }
@@ -572,8 +725,9 @@ int amdk_density(amdk_ctx* ctx, const double* c0, int ldc, const double* occ, in
   AMDK_TRY_HIP(hipMalloc(&dc0, bytes));
   AMDK_TRY_HIP(hipMalloc(&drho, sizeof(double) * nr));
   bands_in(ctx, c0, ldc, nb, dc0);
-  if (ctx->prec == AMDK_PREC_FP64) run_density<double>(ctx, dc0, occ, nb, drho);
-  else run_density<float>(ctx, dc0, occ, nb, drho);
+  prepare_density_weights(ctx, occ, nb);
+  if (ctx->prec == AMDK_PREC_FP64) run_density<double>(ctx, dc0, nb, drho);
+  else run_density<float>(ctx, dc0, nb, drho);
   sync(ctx);
   AMDK_TRY_HIP(hipMemcpy(rho_r, drho, sizeof(double) * nr, hipMemcpyDeviceToHost));
   AMDK_TRY_HIP(hipFree(dc0));

@@ -45,6 +45,7 @@ struct amdk_ctx {
   double omega = 0;
   amdk::Recip rc;
   int prec = AMDK_PREC_FP64;
+  bool fft_generic = false;   // AMDK_FFT=stockham forces the LDS Stockham passes
   int nbmax = 0;
   hipStream_t stream = nullptr;
   amdk::Sphere spw, srho;

@@ -58,6 +58,7 @@ struct Sphere {
   int* d_from_caller = nullptr;  // device [ncaller]
   int* d_conj_caller = nullptr;  // device [ncaller]
   int ncaller = 0;
+  bool planes_contiguous = false;  // each plane's rows are consecutive m2
 };
 
 }  // namespace amdk
