@@ -182,6 +182,17 @@ class KSContext:
         self._chk(self.lib.amdk_scf_step(self.h, istep, L.XC[xc], C.byref(e)))
         return e.as_dict()
 
+    def scf_step_tfor(self, istep, xc, want_forces):
+        """One SCF step; with want_forces also the forces of the pre-update state."""
+        e = L.Energies()
+        f = np.zeros(3 * self.nat) if want_forces else None
+        fp = L.dptr(f) if want_forces else None
+        self._chk(self.lib.amdk_scf_step_tfor(self.h, istep, L.XC[xc], C.byref(e), fp))
+        return e.as_dict(), (f.reshape(-1, 3) if want_forces else None)
+
+    def ortho_resident(self):
+        self._chk(self.lib.amdk_wfn_ortho(self.h))
+
     def scf_forces(self):
         f = np.zeros(3 * self.nat)
         self._chk(self.lib.amdk_scf_forces(self.h, L.dptr(f)))

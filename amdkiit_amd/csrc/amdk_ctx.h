@@ -10,6 +10,7 @@
 #include "fft_passes.h"
 #include "gemm.h"
 #include "dense.h"
+#include "gemm_mfma.h"
 
 #define AMDK_TRY_HIP(expr)                                                         \
   do {                                                                             \
@@ -128,6 +129,7 @@ struct amdk_ctx {
   int nb_state = 0;
   std::vector<double> h_occ_state;
   void* rocblas = nullptr;    // rocblas_handle
+  int chol_lds_max = -1;      // max LDS bytes per workgroup (queried once)
   // allocations to free
   std::vector<void*> owned;
   std::string err;

@@ -415,4 +415,126 @@ __global__ void __launch_bounds__(2 * HR * (A > B ? A : B))
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Dense (density-sphere) passes in FP64: two real fields travel as a + i b.
+
+// z-FFT exp(-i) of full columns of a(r) + i b(r); keep planes |m3| <= R.
+template <int A, int B, int TX>
+__global__ void __launch_bounds__(TX * (A > B ? A : B))
+    k_z_fwd_dense_r(PassGeom g, const double* __restrict__ a, const double* __restrict__ b,
+                    double2* __restrict__ W2, const double2* __restrict__ twg) {
+  constexpr int N = A * B, TM = MaxAB<A, B>::v;
+  extern __shared__ unsigned char smem_raw[];
+  double2* tw = reinterpret_cast<double2*>(smem_raw);
+  double2* lds = tw + N;
+  const int c = threadIdx.x % TX, t = threadIdx.x / TX;
+  const int n1 = g.n1, n2 = g.n2;
+  const int x = blockIdx.x * TX + c, y = blockIdx.y;
+  const bool live = x < n1;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twg[i];
+  __syncthreads();
+  double2 r[TM];
+#pragma unroll
+  for (int k = 0; k < TM; ++k) r[k] = make_double2(0.0, 0.0);
+  if (t < A && live) {
+#pragma unroll
+    for (int k2 = 0; k2 < B; ++k2) {
+      const size_t idx = ((size_t)(t + A * k2) * n2 + y) * n1 + x;
+      r[k2] = make_double2(a[idx], b ? b[idx] : 0.0);
+    }
+  }
+  col_fft<double2, A, B, -1>(r, t, lds, TX, c, tw);
+  if (t < B && live) {
+#pragma unroll
+    for (int j1 = 0; j1 < A; ++j1) {
+      const int kz = signed_freq(t + B * j1, N);
+      if (kz >= -g.R && kz <= g.R) W2[((size_t)(kz + g.R) * n2 + y) * n1 + x] = r[j1];
+    }
+  }
+}
+
+// z-FFT exp(+i) of the planes |m3| <= R to full columns: oa = Re, ob = Im,
+// ov = scale * Re in type T (each may be null).
+template <typename T, int A, int B, int TX>
+__global__ void __launch_bounds__(TX * (A > B ? A : B))
+    k_z_inv_dense_r(PassGeom g, const double2* __restrict__ W2, double* __restrict__ oa, double* __restrict__ ob,
+                    T* __restrict__ ov, double scale, const double2* __restrict__ twg) {
+  constexpr int N = A * B, TM = MaxAB<A, B>::v;
+  extern __shared__ unsigned char smem_raw[];
+  double2* tw = reinterpret_cast<double2*>(smem_raw);
+  double2* lds = tw + N;
+  const int c = threadIdx.x % TX, t = threadIdx.x / TX;
+  const int n1 = g.n1, n2 = g.n2;
+  const int x = blockIdx.x * TX + c, y = blockIdx.y;
+  const bool live = x < n1;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twg[i];
+  __syncthreads();
+  double2 r[TM];
+#pragma unroll
+  for (int k = 0; k < TM; ++k) r[k] = make_double2(0.0, 0.0);
+  if (t < A && live) {
+#pragma unroll
+    for (int k2 = 0; k2 < B; ++k2) {
+      const int kz = signed_freq(t + A * k2, N);
+      if (kz >= -g.R && kz <= g.R) r[k2] = W2[((size_t)(kz + g.R) * n2 + y) * n1 + x];
+    }
+  }
+  col_fft<double2, A, B, +1>(r, t, lds, TX, c, tw);
+  if (t < B && live) {
+#pragma unroll
+    for (int j1 = 0; j1 < A; ++j1) {
+      const size_t idx = ((size_t)(t + B * j1) * n2 + y) * n1 + x;
+      if (oa) oa[idx] = r[j1].x;
+      if (ob) ob[idx] = r[j1].y;
+      if (ov) ov[idx] = (T)(scale * r[j1].x);
+    }
+  }
+}
+
+// x-FFT exp(-i) of direct and mirror rows, unpack two real fields:
+// A = (F(G) + conj F(-G)) / 2, B = (F(G) - conj F(-G)) / 2i, times scale.
+template <int A, int B, int HR>
+__global__ void __launch_bounds__(2 * HR * (A > B ? A : B))
+    k_rows_fwd_unpack_r(PassGeom g, const double2* __restrict__ W1, double2* __restrict__ out, int ldc, double scale,
+                        const double2* __restrict__ twg) {
+  constexpr int N = A * B, TM = MaxAB<A, B>::v, ST = N + 1;
+  extern __shared__ unsigned char smem_raw[];
+  double2* tw = reinterpret_cast<double2*>(smem_raw);
+  double2* lds = tw + N;
+  const int t = threadIdx.x % TM, fr = threadIdx.x / TM;
+  const int hr = fr >> 1, mirror = fr & 1;
+  const int row = blockIdx.x * HR + hr;
+  const bool live = row < g.nrow;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twg[i];
+  RowInfo ri{};
+  if (live) ri = g.rows[row];
+  const bool zero_row = live && ri.slot_m < 0;
+  __syncthreads();
+  double2 r[TM];
+#pragma unroll
+  for (int k = 0; k < TM; ++k) r[k] = make_double2(0.0, 0.0);
+  if (live && t < A && !(zero_row && mirror)) {
+    const double2* in = W1 + (size_t)(mirror ? ri.slot_m : ri.slot_d) * g.n1;
+#pragma unroll
+    for (int k2 = 0; k2 < B; ++k2) r[k2] = in[t + A * k2];
+  }
+  col_fft<double2, A, B, -1>(r, t, lds, 1, fr * ST, tw);
+  if (t < B) {
+#pragma unroll
+    for (int j1 = 0; j1 < A; ++j1) lds[fr * ST + t + B * j1] = r[j1];
+  }
+  __syncthreads();
+  if (live) {
+    const double2* rd = lds + (2 * hr) * ST;
+    const double2* rm = zero_row ? rd : rd + ST;
+    for (int e = t + mirror * TM; e < ri.cnt; e += 2 * TM) {
+      const int m1 = ri.m1_0 + e;
+      const double2 P = rd[wrap(m1, N)], M = rm[wrap(-m1, N)];
+      out[ri.start + e] = make_double2(0.5 * scale * (P.x + M.x), 0.5 * scale * (P.y - M.y));
+      out[(size_t)ldc + ri.start + e] = make_double2(0.5 * scale * (P.y + M.y), -0.5 * scale * (P.x - M.x));
+    }
+  }
+}
+
 }  // namespace amdk

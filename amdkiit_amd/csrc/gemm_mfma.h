@@ -1,0 +1,237 @@
+// Band-space GEMMs on the matrix cores: FP64 (v_mfma_f64_16x16x4f64) or
+// FP32 (v_mfma_f32_16x16x4f32) arithmetic on FP64 band arrays.
+//
+// Complex band arrays are read as real arrays of 2*ldc doubles per band (the
+// Gamma-point real form of OVLAP_D: K = 2*N_pw reals), so every product is a
+// real GEMM:
+//   ovl:  part_s(m, n) = sum_{k in split s} X_m[k] Y_n[k]     (TN, split over K)
+//   upd:  Y_n[k] = beta Y_n[k] + alpha sum_m Q(m, n) X_m[k]    (NN)
+// Both 16x16x4 instructions take the same operands (wave64): lane l supplies
+// A(l%16, l/16) and B(l/16, l%16).  They spread the 16x16 result differently
+// over the four accumulator registers v of lane l: the FP32 form holds
+// D(4*(l/16) + v, l%16), the FP64 form D(4*v + l/16, l%16) (one row group per
+// register), see acc_row().  The FP32 variant rounds the operands to FP32
+// while staging them, accumulates one 32-deep K tile in FP32 and adds every
+// tile into FP64 accumulators, so the rounding error stays at FP32 level
+// relative to the tile sums whatever the total K.  Accumulation order is
+// fixed, so results are deterministic.
+#pragma once
+#include "fft_core.h"
+
+namespace amdk {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ dbl4 mfma16(double a, double b, dbl4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ flt4 mfma16(float a, float b, flt4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// row of the 16x16 result held by accumulator register v of a lane with l/16 = lk
+template <typename TC>
+__device__ __forceinline__ int acc_row(int lk, int v) {
+  return sizeof(TC) == 8 ? 4 * v + lk : 4 * lk + v;
+}
+
+template <typename TC> struct Acc4;
+template <> struct Acc4<double> { typedef dbl4 type; };
+template <> struct Acc4<float> { typedef flt4 type; };
+
+__device__ __forceinline__ void acc_flush(dbl4& hi, dbl4& lo) {}  // FP64: accumulate in place
+__device__ __forceinline__ void acc_flush(dbl4& hi, flt4& lo) {
+  hi[0] += (double)lo[0];
+  hi[1] += (double)lo[1];
+  hi[2] += (double)lo[2];
+  hi[3] += (double)lo[3];
+  lo = (flt4){0.f, 0.f, 0.f, 0.f};
+}
+
+constexpr int OVM_T = 64, OVM_KT = 32;
+
+// Block tile 64 x 64 of the result, 4 waves of 32 x 32 (2 x 2 MFMA tiles).
+template <typename TC>
+__global__ void __launch_bounds__(256) k_ovl_mfma(const double* __restrict__ X, int ldx, int M,
+                                                  const double* __restrict__ Y, int ldy, int N, int K,
+                                                  int kchunk, double* __restrict__ part) {
+  typedef typename Acc4<TC>::type AccT;
+  constexpr bool F64 = sizeof(TC) == 8;
+  __shared__ TC xs[OVM_T][OVM_KT + 1];
+  __shared__ TC ys[OVM_T][OVM_KT + 1];
+  const int m0 = blockIdx.x * OVM_T, n0 = blockIdx.y * OVM_T, split = blockIdx.z;
+  const int kb = split * kchunk, ke = min(K, kb + kchunk);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  dbl4 acc[2][2];
+  AccT tac[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      acc[a][b] = (dbl4){0.0, 0.0, 0.0, 0.0};
+      tac[a][b] = (AccT){0, 0, 0, 0};
+    }
+  for (int k0 = kb; k0 < ke; k0 += OVM_KT) {
+    for (int e = threadIdx.x; e < OVM_T * OVM_KT; e += 256) {
+      const int r = e / OVM_KT, kk = e % OVM_KT, k = k0 + kk;
+      xs[r][kk] = (TC)((m0 + r < M && k < ke) ? X[(size_t)(m0 + r) * ldx + k] : 0.0);
+      ys[r][kk] = (TC)((n0 + r < N && k < ke) ? Y[(size_t)(n0 + r) * ldy + k] : 0.0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < OVM_KT; ks += 4) {
+      const int kk = ks + lk;
+      const TC a0 = xs[wm + li][kk], a1 = xs[wm + 16 + li][kk];
+      const TC b0 = ys[wn + li][kk], b1 = ys[wn + 16 + li][kk];
+      if constexpr (F64) {
+        acc[0][0] = mfma16(a0, b0, acc[0][0]);
+        acc[0][1] = mfma16(a0, b1, acc[0][1]);
+        acc[1][0] = mfma16(a1, b0, acc[1][0]);
+        acc[1][1] = mfma16(a1, b1, acc[1][1]);
+      } else {
+        tac[0][0] = mfma16(a0, b0, tac[0][0]);
+        tac[0][1] = mfma16(a0, b1, tac[0][1]);
+        tac[1][0] = mfma16(a1, b0, tac[1][0]);
+        tac[1][1] = mfma16(a1, b1, tac[1][1]);
+      }
+    }
+    if constexpr (!F64) {
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc_flush(acc[a][b], tac[a][b]);
+    }
+    __syncthreads();
+  }
+  double* P = part + (size_t)split * M * N;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int m = m0 + wm + 16 * a + acc_row<TC>(lk, v);
+        const int n = n0 + wn + 16 * b + li;
+        if (m < M && n < N) P[(size_t)n * M + m] = acc[a][b][v];
+      }
+}
+
+constexpr int UPM_MC = 32, UPM_NB = 128;
+
+// Block: 64 reals of k (4 waves x 16) x 128 columns n (8 MFMA tiles per wave).
+template <typename TC>
+__global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X, int ldx, int M,
+                                                  const double* __restrict__ Q, int ldq,
+                                                  double* __restrict__ Y, int ldy, int N, int K,
+                                                  double alpha, double beta) {
+  typedef typename Acc4<TC>::type AccT;
+  constexpr bool F64 = sizeof(TC) == 8;
+  __shared__ TC qs[UPM_MC][UPM_NB + 1];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int k = blockIdx.x * 64 + wave * 16 + li;
+  const int n0 = blockIdx.y * UPM_NB;
+  dbl4 acc[8];
+  AccT tac[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    acc[t] = (dbl4){0.0, 0.0, 0.0, 0.0};
+    tac[t] = (AccT){0, 0, 0, 0};
+  }
+  for (int mb = 0; mb < M; mb += UPM_MC) {
+    for (int e = threadIdx.x; e < UPM_MC * UPM_NB; e += 256) {
+      const int i = e / UPM_NB, j = e % UPM_NB;
+      qs[i][j] = (TC)((mb + i < M && n0 + j < N) ? Q[(size_t)(n0 + j) * ldq + mb + i] : 0.0);
+    }
+    __syncthreads();
+    const int mend = min(UPM_MC, M - mb);
+    for (int ms = 0; ms < mend; ms += 4) {
+      const int m = mb + ms + lk;
+      const TC b = (TC)((m < M && k < K) ? X[(size_t)m * ldx + k] : 0.0);
+      if constexpr (F64) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) acc[t] = mfma16(qs[ms + lk][16 * t + li], b, acc[t]);
+      } else {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) tac[t] = mfma16(qs[ms + lk][16 * t + li], b, tac[t]);
+      }
+    }
+    if constexpr (!F64) {
+#pragma unroll
+      for (int t = 0; t < 8; ++t) acc_flush(acc[t], tac[t]);
+    }
+    __syncthreads();
+  }
+  if (k < K) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int n = n0 + 16 * t + acc_row<TC>(lk, v);
+        if (n < N) {
+          double* y = Y + (size_t)n * ldy + k;
+          *y = (beta != 0.0 ? beta * *y : 0.0) + alpha * acc[t][v];
+        }
+      }
+  }
+}
+
+// Cholesky S = U^T U and U^-1 in LDS by one block (FP64).  A is n x n
+// column-major; on return its upper triangle holds U^-1 and the strict lower
+// triangle is zero.  info = k+1 if the k-th leading minor is not positive.
+__global__ void __launch_bounds__(1024) k_chol_inv(double* __restrict__ A, int n, int* __restrict__ info) {
+  extern __shared__ double smem_d[];
+  double* L = smem_d;            // n*n
+  double* tmp = smem_d + n * n;  // n
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int e = tid; e < n * n; e += nt) L[e] = A[e];
+  if (tid == 0) *info = 0;
+  __syncthreads();
+  int bad = 0;
+  // right-looking factorisation with deferred row scaling: after step k the
+  // trailing block holds S_kk... minus sum_{p<k} a_pi a_pj / a_pp
+  for (int k = 0; k < n; ++k) {
+    const double akk = L[k + (size_t)k * n];
+    if (!(akk > 0.0)) {
+      bad = k + 1;
+      break;
+    }
+    const double inv = 1.0 / akk;
+    const int m = n - k - 1;
+    for (int e = tid; e < m * m; e += nt) {
+      const int i = k + 1 + e % m, j = k + 1 + e / m;
+      if (i <= j) L[i + (size_t)j * n] -= L[k + (size_t)i * n] * L[k + (size_t)j * n] * inv;
+    }
+    __syncthreads();
+  }
+  if (bad) {
+    if (tid == 0) *info = bad;
+    return;
+  }
+  for (int k = tid; k < n; k += nt) tmp[k] = sqrt(L[k + (size_t)k * n]);
+  __syncthreads();
+  for (int e = tid; e < n * n; e += nt) {
+    const int i = e % n, j = e / n;
+    L[e] = (i <= j) ? L[e] / tmp[i] : 0.0;
+  }
+  __syncthreads();
+  // U^-1 column by column: X(i,j) = -(sum_{k=i}^{j-1} X(i,k) U(k,j)) / U(j,j)
+  for (int j = 0; j < n; ++j) {
+    const double ujj = L[j + (size_t)j * n];
+    for (int i = tid; i < j; i += nt) {
+      double s = 0.0;
+      for (int k = i; k < j; ++k) s += L[i + (size_t)k * n] * L[k + (size_t)j * n];
+      tmp[i] = -s / ujj;
+    }
+    __syncthreads();
+    for (int i = tid; i < j; i += nt) L[i + (size_t)j * n] = tmp[i];
+    if (tid == 0) L[j + (size_t)j * n] = 1.0 / ujj;
+    __syncthreads();
+  }
+  for (int e = tid; e < n * n; e += nt) A[e] = L[e];
+}
+
+}  // namespace amdk

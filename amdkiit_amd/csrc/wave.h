@@ -194,6 +194,34 @@ __global__ void k_precon(const V* __restrict__ C2, V* __restrict__ P, int ldc, i
   }
 }
 
+// Fused: P = f^-1 K^-1 C2 (zero padding) and partial sums of sum_b w_b <P_b|P_b>.
+template <typename V>
+__global__ void __launch_bounds__(256) k_precon_norm(const V* __restrict__ C2, V* __restrict__ P, int ldc, int npw,
+                                                     int nb, const double* __restrict__ vpp,
+                                                     const double* __restrict__ finv, const double* __restrict__ w,
+                                                     double* __restrict__ part) {
+  __shared__ double sh[4];
+  double s = 0;
+  const size_t tot = (size_t)nb * ldc;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+    const int b = (int)(i / ldc), g = (int)(i % ldc);
+    V o;
+    if (g < npw) {
+      const double f = finv[b] * vpp[g];
+      o.x = f * (double)C2[i].x;
+      o.y = f * (double)C2[i].y;
+      const double a2 = (double)o.x * o.x + (double)o.y * o.y;
+      s += w[b] * (g == 0 ? (double)o.x * o.x : 2.0 * a2);
+    } else {
+      o.x = 0;
+      o.y = 0;
+    }
+    P[i] = o;
+  }
+  const double r = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
 // scalars: st[0] = ||P||^2 of this step (input), st[1] = previous ||P||^2,
 // st[2] = gamma out.  istep is 1-based within one optimisation.
 __global__ void k_pcg_scalar(double* st, int istep) {

@@ -38,7 +38,8 @@ module amdk_hip
   public :: amdk_ctx_create, amdk_ctx_destroy, amdk_set_vks, amdk_vpsi, amdk_density, &
             amdk_set_species, amdk_set_positions, amdk_nproj, amdk_fnl, amdk_fnonloc, &
             amdk_overlap, amdk_lagrange, amdk_ortho, amdk_vks, amdk_forces, &
-            amdk_wfn_load, amdk_wfn_store, amdk_scf_step, amdk_scf_forces
+            amdk_wfn_load, amdk_wfn_store, amdk_scf_step, amdk_scf_forces, amdk_scf_step_tfor, &
+            amdk_wfn_ortho
   public :: amdk_check, vpsi_hip, fnonloc_hip, fnl_hip, ovlap_d_hip, forces_hip, ortho_gs_hip, &
             eval_density_hip
 
@@ -164,6 +165,17 @@ module amdk_hip
       type(c_ptr), value :: ctx
       integer(c_int), value :: istep, xc
       type(amdk_energies), intent(out) :: e
+    end function
+    integer(c_int) function amdk_scf_step_tfor(ctx, istep, xc, e, f) bind(C, name="amdk_scf_step_tfor")
+      import :: c_int, c_ptr, c_double, amdk_energies
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: istep, xc
+      type(amdk_energies), intent(out) :: e
+      real(c_double), intent(out) :: f(*)
+    end function
+    integer(c_int) function amdk_wfn_ortho(ctx) bind(C, name="amdk_wfn_ortho")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
     end function
     integer(c_int) function amdk_scf_forces(ctx, f) bind(C, name="amdk_scf_forces")
       import :: c_int, c_ptr, c_double

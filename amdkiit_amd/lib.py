@@ -76,6 +76,8 @@ SIGNATURES = {
     "amdk_wfn_store": (_i, [_vp, _dp, _i]),
     "amdk_scf_step": (_i, [_vp, _i, _i, C.POINTER(Energies)]),
     "amdk_scf_forces": (_i, [_vp, _dp]),
+    "amdk_scf_step_tfor": (_i, [_vp, _i, _i, C.POINTER(Energies), _dp]),
+    "amdk_wfn_ortho": (_i, [_vp]),
     "amdk_bands_to_dev": (_i, [_vp, _dp, _i, _i, _vp]),
     "amdk_bands_from_dev": (_i, [_vp, _vp, _i, _dp, _i]),
     "amdk_sphere_info": (_i, [_vp, _i, _ip]),

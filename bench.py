@@ -131,6 +131,7 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the 64 H2O secondary measurement")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 512 MiB device-to-device copy (PMC byte-counter calibration)")
@@ -179,16 +180,40 @@ def main():
     dominant = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"])
 
     fp64 = None
+    ctx.close()
     if not a.no_fp64 and a.precision == "mixed":
-        ctx.close()
-        steps64 = max(3, a.steps // 2)
-        ctx64, dt64, e64, prof64, _ = run_gpu(s, "fp64", steps64, a.warmup, c0, profile_steps=1)
-        fp64 = {"steps_per_s": steps64 / dt64, "ms_per_step": dt64 / steps64 * 1e3,
-                "mixed_speedup": value / (steps64 / dt64)}
+        ctx64, dt64, e64, prof64, _ = run_gpu(s, "fp64", a.steps, a.warmup, c0, profile_steps=1)
+        fp64 = {"steps_per_s": a.steps / dt64, "ms_per_step": dt64 / a.steps * 1e3,
+                "mixed_speedup": value / (a.steps / dt64),
+                "etot_fp64": e64["etot"],
+                "etot_diff_per_atom": abs(e["etot"] - e64["etot"]) / s.nat,
+                "note": "same seeded start and step count (1 optimisation) in both precisions"}
         ctx64.close()
-        log(f"[bench] fp64: {fp64['ms_per_step']:.3f} ms/step; mixed speedup {fp64['mixed_speedup']:.2f}x")
-    else:
-        ctx.close()
+        log(f"[bench] fp64: {fp64['ms_per_step']:.3f} ms/step; mixed speedup {fp64['mixed_speedup']:.2f}x; "
+            f"|dE|/atom={fp64['etot_diff_per_atom']:.2e}")
+
+    secondary = None
+    if not a.no_secondary and a.config == "h2o_32_pbe":
+        log("[bench] secondary: 64 H2O")
+        s2 = systems.CONFIGS["h2o_64_pbe"]()
+        rng2 = np.random.default_rng(2025)
+        c02 = (rng2.random((s2.nbands, s2.pw.ngpw)) - 0.5 + 1j * (rng2.random((s2.nbands, s2.pw.ngpw)) - 0.5)) / (1.0 + s2.pw.g2_pw)
+        c02[:, 0] = c02[:, 0].real
+        st2 = max(5, a.steps // 2)
+        cx, d2, e2, p2, _ = run_gpu(s2, "mixed", st2, a.warmup, c02, profile_steps=2)
+        hp2 = sum(p2[k][0] / p2[k][1] for k in HPSI_KERNELS if k in p2 and p2[k][1])
+        hb2 = hpsi_bytes(s2.pw, s2.nbands, "mixed")
+        cx.close()
+        c64, d64, e64b, _, _ = run_gpu(s2, "fp64", st2, a.warmup, c02, profile_steps=1)
+        c64.close()
+        secondary = {"workload": CONFIGS["h2o_64_pbe"], "steps": st2,
+                     "mixed_steps_per_s": st2 / d2, "mixed_ms_per_step": d2 / st2 * 1e3,
+                     "fp64_steps_per_s": st2 / d64, "mixed_speedup": d64 / d2,
+                     "hpsi_ms": hp2, "hpsi_GBps": hb2 / (hp2 * 1e-3) / 1e9,
+                     "hpsi_frac_hbm": hb2 / (hp2 * 1e-3) / 1e9 / HBM_PEAK,
+                     "etot_diff_per_atom": abs(e2["etot"] - e64b["etot"]) / s2.nat}
+        log(f"[bench] 64 H2O: mixed {secondary['mixed_ms_per_step']:.2f} ms/step, fp64 speedup "
+            f"{secondary['mixed_speedup']:.2f}x, H|psi> {secondary['hpsi_GBps']:.0f} GB/s")
 
     cpu = None
     if not a.no_cpu:
@@ -219,6 +244,7 @@ def main():
         "step_ms_by_region": step_breakdown,
         "energy": {"etot": e["etot"], "gemax": e["gemax"], "nelec": e["nelec"], "after_steps": it},
         "fp64_gpu": fp64,
+        "secondary_64h2o": secondary,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

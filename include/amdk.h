@@ -154,8 +154,15 @@ int amdk_forces(amdk_ctx* ctx, const double* c0, int ldc, const double* occ, int
 int amdk_wfn_load(amdk_ctx* ctx, const double* c0, int ldc, const double* occ, int nb);
 int amdk_wfn_store(amdk_ctx* ctx, double* c0, int ldc);
 int amdk_scf_step(amdk_ctx* ctx, int istep, int xc, amdk_energies* e);
+/* As amdk_scf_step, and with f non-null also the nuclear forces [3*nat] of
+ * this step's state before its wavefunction update: the TFOR branch of
+ * wfn_opt (E_SR + POTFOR + NLFOR, do_opt.f90:97-106). */
+int amdk_scf_step_tfor(amdk_ctx* ctx, int istep, int xc, amdk_energies* e, double* f);
 /* Forces for the resident C_0 and the rho(G) of the last SCF step. */
 int amdk_scf_forces(amdk_ctx* ctx, double* f);
+/* Cholesky-orthonormalise the resident C_0 (ortho_gs(c_0) after a geometry
+ * update, molecular_dynamics.f90:108, do_opt.f90:259). */
+int amdk_wfn_ortho(amdk_ctx* ctx);
 
 /* Transfer helpers between caller-layout FP64 host arrays and the internal
  * device layout (no reference counterpart: they replace nothing, they keep

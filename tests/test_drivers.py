@@ -1,0 +1,80 @@
+"""Host drivers (BO-MD velocity Verlet, L-BFGS) on analytic potentials."""
+import math
+
+import numpy as np
+
+from amdkiit_amd import drivers as D
+
+
+def spring_provider(k, x0, d0=1.5):
+    """Translation-invariant pair springs 0.5 k (|r_ij| - d0)^2 plus a weak
+    anharmonic term, so the net force is zero as for a periodic cell."""
+    n = len(x0)
+
+    def p(tau):
+        e = 0.0
+        f = np.zeros_like(tau)
+        for i in range(n):
+            for j in range(i + 1, n):
+                r = tau[i] - tau[j]
+                d = float(np.linalg.norm(r))
+                x = d - d0
+                e += 0.5 * k * x * x + 0.1 * k * x ** 4
+                g = (k * x + 0.4 * k * x ** 3) * r / d
+                f[i] -= g
+                f[j] += g
+        return e, f
+    return p
+
+
+def test_maxwell_boltzmann_temperature_and_momentum():
+    m = np.array([16.0, 1.0, 1.0, 16.0, 1.0, 1.0]) * D.AMU_AU
+    v = D.maxwell_boltzmann(m, 300.0, seed=5)
+    assert np.abs((m[:, None] * v).sum(0)).max() < 1e-12
+    t = (m[:, None] * v * v).sum() / ((3 * len(m) - 3) * D.KBOLTZ)
+    assert abs(t - 300.0) < 1e-9
+    assert not np.allclose(v[0], v[3])  # independent draws per atom
+
+
+def test_velocity_verlet_conserves_energy_and_is_reversible():
+    rng = np.random.default_rng(0)
+    x0 = rng.normal(size=(4, 3))
+    m = np.array([1.0, 2.0, 3.0, 4.0]) * D.AMU_AU
+    k = 0.3
+    prov = spring_provider(k, x0)
+    tau = x0 + 0.1 * rng.normal(size=(4, 3))
+    v = D.maxwell_boltzmann(m, 300.0, seed=1)
+    h, drift, tau1, v1 = D.bomd(prov, tau, m, v, dt_fs=0.1, nsteps=400)
+    # second-order integrator: bounded energy error, no secular drift
+    span = h[:, 0].max() - h[:, 0].min()
+    assert span > 1e-4  # the system actually moves
+    assert np.abs(h[:, 2] - h[0, 2]).max() < 1e-2 * span
+    assert abs(drift) < 1e-3
+    # time reversal returns to the start (COM shift keeps it fixed)
+    _, _, tau2, v2 = D.bomd(prov, tau1, m, -v1, dt_fs=0.1, nsteps=400)
+    com = lambda t: (m[:, None] * t).sum(0) / m.sum()
+    assert np.abs((tau2 - com(tau2)) - (tau - com(tau))).max() < 1e-8
+
+
+def test_lbfgs_minimises_quadratic_and_rosenbrock():
+    rng = np.random.default_rng(1)
+    A = rng.normal(size=(6, 6))
+    H = A @ A.T + 6 * np.eye(6)
+    xs = rng.normal(size=6)
+
+    def quad(x):
+        d = x - xs
+        return 0.5 * float(d @ H @ d), -(H @ d)
+
+    x, hist = D.lbfgs(quad, np.zeros(6), gtol=1e-8, maxiter=60)
+    assert np.abs(x - xs).max() < 1e-6
+
+    def rosen(x):
+        a, b = x
+        e = (1 - a) ** 2 + 100 * (b - a * a) ** 2
+        g = np.array([-2 * (1 - a) - 400 * a * (b - a * a), 200 * (b - a * a)])
+        return e, -g
+
+    x, hist = D.lbfgs(rosen, np.array([-1.2, 1.0]), gtol=1e-6, maxiter=200)
+    assert np.abs(x - 1.0).max() < 1e-4
+    assert all(hist[i + 1][0] <= hist[i][0] + 1e-12 for i in range(len(hist) - 1))

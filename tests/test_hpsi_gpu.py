@@ -12,7 +12,12 @@ pytestmark = pytest.mark.gpu
 CASES = [
     ("small_odd_bands", lambda: small_setup(), 5),
     ("noncubic_grid", lambda: PWSetup(Cell(np.diag([9.0, 10.0, 11.0])), 10.0), 4),
+    ("generic_lengths_16_12_10", lambda: PWSetup(Cell(np.diag([8.0, 6.5, 5.2])), 5.0, n=(16, 12, 10)), 3),
     ("cfg1_shape_108", lambda: PWSetup(Cell.cubic(18.9), 70.0), 4),
+    ("cfg3_shape_72", lambda: PWSetup(Cell.cubic(20.5224), 30.0), 3),
+    ("cfg2_shape_100", lambda: PWSetup(Cell.cubic(18.63), 70.0), 2),
+    ("cfg4_shape_144", lambda: PWSetup(Cell.cubic(23.47), 80.0), 2),
+    ("cfg5_shape_180", lambda: PWSetup(Cell.cubic(29.57), 80.0), 2),
 ]
 
 
