@@ -44,6 +44,22 @@ void* dev_alloc(amdk_ctx* c, size_t bytes) {
   return p;
 }
 
+// Device buffers handed in by the caller may have been written on any
+// stream; the library stream does not order after them, so wait for the device.
+static void caller_ready() { AMDK_TRY_HIP(hipDeviceSynchronize()); }
+
+// Host <-> device copies go through the library stream (which does not
+// synchronise with the null stream), after the zero fill of dev_alloc and any
+// queued kernels, and are complete when these return.
+void upload(amdk_ctx* c, void* dst, const void* src, size_t bytes) {
+  AMDK_TRY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  AMDK_TRY_HIP(hipStreamSynchronize(c->stream));
+}
+void download(amdk_ctx* c, void* dst, const void* src, size_t bytes) {
+  AMDK_TRY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  AMDK_TRY_HIP(hipStreamSynchronize(c->stream));
+}
+
 size_t cbytes(const amdk_ctx* c) { return c->prec == AMDK_PREC_FP64 ? sizeof(double2) : sizeof(float2); }
 size_t sbytes(const amdk_ctx*) { return sizeof(double2); }
 
@@ -171,11 +187,11 @@ static void build_sphere(amdk_ctx* c, const int* mill, int ncaller, Sphere& S) {
   S.plane_off = (int*)dev_alloc(c, sizeof(int) * (S.nplane + 1));
   S.d_from_caller = (int*)dev_alloc(c, sizeof(int) * ncaller);
   S.d_conj_caller = (int*)dev_alloc(c, sizeof(int) * ncaller);
-  AMDK_TRY_HIP(hipMemcpy(S.rows, S.h_rows.data(), sizeof(RowInfo) * S.nrow, hipMemcpyHostToDevice));
-  AMDK_TRY_HIP(hipMemcpy(S.slot_m2, S.h_slot_m2.data(), sizeof(int) * S.nslot, hipMemcpyHostToDevice));
-  AMDK_TRY_HIP(hipMemcpy(S.plane_off, S.h_plane_off.data(), sizeof(int) * (S.nplane + 1), hipMemcpyHostToDevice));
-  AMDK_TRY_HIP(hipMemcpy(S.d_from_caller, S.h_from_caller.data(), sizeof(int) * ncaller, hipMemcpyHostToDevice));
-  AMDK_TRY_HIP(hipMemcpy(S.d_conj_caller, S.h_conj_caller.data(), sizeof(int) * ncaller, hipMemcpyHostToDevice));
+  upload(c, S.rows, S.h_rows.data(), sizeof(RowInfo) * S.nrow);
+  upload(c, S.slot_m2, S.h_slot_m2.data(), sizeof(int) * S.nslot);
+  upload(c, S.plane_off, S.h_plane_off.data(), sizeof(int) * (S.nplane + 1));
+  upload(c, S.d_from_caller, S.h_from_caller.data(), sizeof(int) * ncaller);
+  upload(c, S.d_conj_caller, S.h_conj_caller.data(), sizeof(int) * ncaller);
 }
 
 static PassGeom geom_of(const amdk_ctx* c, const Sphere& S) {
@@ -592,8 +608,8 @@ int amdk_ctx_create(const amdk_desc* d, amdk_ctx** out) {
       }
       ctx->tw_d[ax] = dev_alloc(ctx, sizeof(double2) * n);
       ctx->tw_f[ax] = dev_alloc(ctx, sizeof(float2) * n);
-      AMDK_TRY_HIP(hipMemcpy(ctx->tw_d[ax], td.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
-      AMDK_TRY_HIP(hipMemcpy(ctx->tw_f[ax], tf.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+      upload(ctx, ctx->tw_d[ax], td.data(), sizeof(double2) * n);
+      upload(ctx, ctx->tw_f[ax], tf.data(), sizeof(float2) * n);
     }
     const size_t nr = (size_t)d->n1 * d->n2 * d->n3;
     ctx->vr = dev_alloc(ctx, (ctx->prec == AMDK_PREC_FP64 ? sizeof(double) : sizeof(float)) * nr);
@@ -668,6 +684,7 @@ int amdk_set_vks(amdk_ctx* ctx, const double* v_r) {
 
 int amdk_vpsi_dev(amdk_ctx* ctx, const void* c0, void* c2, const double* occ, int nb) {
   API_BEGIN
+  caller_ready();
   check_nb(ctx, nb);
   if (!c0 || !c2 || !occ) throw Error(AMDK_ERR_ARG, "null pointer");
   if (!ctx->have_vks) throw Error(AMDK_ERR_STATE, "V_KS not set (amdk_set_vks)");
@@ -703,6 +720,7 @@ int amdk_vpsi(amdk_ctx* ctx, const double* c0, int ldc, double* c2, const double
 
 int amdk_density_dev(amdk_ctx* ctx, const void* c0, const double* occ, int nb, double* rho_r_dev) {
   API_BEGIN
+  caller_ready();
   check_nb(ctx, nb);
   if (!c0 || !occ || !rho_r_dev) throw Error(AMDK_ERR_ARG, "null pointer");
   prepare_density_weights(ctx, occ, nb);
@@ -729,7 +747,7 @@ int amdk_density(amdk_ctx* ctx, const double* c0, int ldc, const double* occ, in
   if (ctx->prec == AMDK_PREC_FP64) run_density<double>(ctx, dc0, nb, drho);
   else run_density<float>(ctx, dc0, nb, drho);
   sync(ctx);
-  AMDK_TRY_HIP(hipMemcpy(rho_r, drho, sizeof(double) * nr, hipMemcpyDeviceToHost));
+  download(ctx, rho_r, drho, sizeof(double) * nr);
   AMDK_TRY_HIP(hipFree(dc0));
   AMDK_TRY_HIP(hipFree(drho));
   API_END
@@ -746,6 +764,7 @@ int amdk_bands_to_dev(amdk_ctx* ctx, const double* c, int ldc, int nb, void* dst
 
 int amdk_bands_from_dev(amdk_ctx* ctx, const void* src, int nb, double* c, int ldc) {
   API_BEGIN
+  caller_ready();
   check_nb(ctx, nb);
   if (!c || !src) throw Error(AMDK_ERR_ARG, "null pointer");
   if (ldc < ctx->spw.npw) throw Error(AMDK_ERR_ARG, "ldc < ngpw");

@@ -22,10 +22,12 @@
  *  - *_dev entry points take device pointers in the context's internal
  *    layout (plane-wave order of amdk_pw_order, leading dimension
  *    amdk_ldc(), element type double2) so that wavefunctions stay resident
- *    in HBM.  Coefficients are stored in FP64 in both precision modes; the
- *    mode selects the arithmetic of the FFTs and of V(r) psi(r)
- *    (AMDK_PREC_MIXED: FP32) while overlaps, Cholesky, energies and the
- *    density-grid potential stay FP64.
+ *    in HBM.  They wait for the whole device before reading caller buffers,
+ *    which may have been written on any stream.  Coefficients are stored in
+ *    FP64 in both precision modes; AMDK_PREC_MIXED runs the FFTs, V(r) psi(r)
+ *    and the projector and Lagrange-multiplier GEMMs in FP32 (FP64 sums of
+ *    32-deep FP32 tiles), while the orthonormalisation (overlap, Cholesky,
+ *    C U^-1), energies, forces' sums and the density-grid potential stay FP64.
  */
 #ifndef AMDK_H
 #define AMDK_H

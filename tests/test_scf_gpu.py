@@ -176,3 +176,29 @@ def test_scf_reproduces_recorded_reference_energy(nmol, prec):
     assert abs(e["eself"] - ref["eself"]) < 1e-9
     assert abs(e["nelec"] - s.occ.sum()) < 1e-10 * s.occ.sum()
     ctx.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_large_context_tables_and_first_step(prec):
+    """64 H2O at 80 Ry (144^3, 78,098 plane waves, 256 bands), created after a
+    smaller context was freed: the uploaded tables (Miller indices, twnl,
+    projector lists) must be in place before any kernel reads them."""
+    small = KSContext(*(lambda s: (s.pw, s.nbands))(SMALL["h2o2_pz"]()), prec)
+    small.close()
+    s = systems.CONFIGS["h2o_64_pbe"]()
+    print(f"case=h2o_64 prec={prec} n={s.pw.n} npw={s.pw.ngpw} nb={s.nbands}", flush=True)
+    ctx = KSContext(s.pw, s.nbands, prec)
+    ctx.set_species(s.tables)
+    ctx.set_positions(s.sp, s.tau)
+    orc = KSOracle(s.pw, s.tables, s.sp, s.tau, s.xc, workers=8)
+    c = ctx.ortho(random_bands(s.pw, s.nbands, seed=12))
+    nl = ctx.fnl(c)
+    assert rel(nl, orc.fnl(c)) < TOL[prec]
+    ctx.wfn_load(c, s.occ)
+    e = ctx.scf_step(1, s.xc)
+    ekin = float(np.sum(s.occ[:, None] * s.pw.g2_pw[None, :] * np.abs(c) ** 2))
+    print(f"  ekin={e['ekin']:.10f} ref={ekin:.10f} eh={e['eh']:.6e} enl={e['enl']:.10f}", flush=True)
+    assert abs(e["ekin"] - ekin) < 1e-10 * ekin
+    assert abs(e["enl"] - orc.e_nl(orc.fnl(c), s.occ)) < 1e-6 * abs(e["enl"])
+    assert abs(e["nelec"] - s.occ.sum()) < 1e-10 * s.occ.sum()
+    ctx.close()
