@@ -113,6 +113,7 @@ struct amdk_ctx {
   double* d_gpart = nullptr;  // split-K partials
   size_t gpart_cap = 0;
   double* d_S = nullptr;      // nbmax x nbmax
+  double* d_Ut = nullptr;     // nbmax x nbmax: Cholesky factor, row-major
   double* d_nl = nullptr;     // P x nbmax
   double* d_Q = nullptr;      // max(P, nbmax) x nbmax
   double* d_finv = nullptr;   // 1/f for PCG

@@ -58,16 +58,27 @@ __global__ void __launch_bounds__(256) k_ovl_partial(const V* __restrict__ X, in
     }
 }
 
-// A = alpha * (2 sum_s part_s - x0 y0) (+ beta A when accumulate)
+// A = alpha * (2 sum_s part_s - x0 y0).  A block sums the split-K partials
+// of 64 consecutive outputs with 4 interleaved groups of splits, combined in a
+// fixed order (deterministic).
 template <typename V>
-__global__ void k_ovl_reduce(const double* __restrict__ part, int nsplit, int M, int N,
-                             const V* __restrict__ X, int ldx, const V* __restrict__ Y, int ldy,
-                             double alpha, double* __restrict__ A, int lda, int g0) {
-  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < M * N; idx += gridDim.x * blockDim.x) {
-    const int m = idx % M, n = idx / M;
-    double s = 0;
-    for (int p = 0; p < nsplit; ++p) s += part[(size_t)p * M * N + idx];
-    double v = 2.0 * s;
+__global__ void __launch_bounds__(256) k_ovl_reduce(const double* __restrict__ part, int nsplit, int M, int N,
+                                                    const V* __restrict__ X, int ldx, const V* __restrict__ Y, int ldy,
+                                                    double alpha, double* __restrict__ A, int lda, int g0) {
+  __shared__ double sh[4][64];
+  const int l = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  const size_t MN = (size_t)M * N;
+  const size_t o = (size_t)blockIdx.x * 64 + l;
+  double s = 0;
+  if (o < MN) {
+#pragma unroll 4
+    for (int p = grp; p < nsplit; p += 4) s += part[(size_t)p * MN + o];
+  }
+  sh[grp][l] = s;
+  __syncthreads();
+  if (grp == 0 && o < MN) {
+    const int m = (int)(o % M), n = (int)(o / M);
+    double v = 2.0 * (((sh[0][l] + sh[1][l]) + sh[2][l]) + sh[3][l]);
     if (g0) v -= (double)X[(size_t)m * ldx].x * (double)Y[(size_t)n * ldy].x;
     A[(size_t)n * lda + m] = alpha * v;
   }

@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X, 
   }
   for (int mb = 0; mb < M; mb += UPM_MC) {
     for (int e = threadIdx.x; e < UPM_MC * UPM_NB; e += 256) {
-      const int i = e / UPM_NB, j = e % UPM_NB;
+      const int i = e % UPM_MC, j = e / UPM_MC;  // m fastest: contiguous in a column of Q
       qs[i][j] = (TC)((mb + i < M && n0 + j < N) ? Q[(size_t)(n0 + j) * ldq + mb + i] : 0.0);
     }
     __syncthreads();
@@ -179,10 +179,12 @@ __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X, 
   }
 }
 
-// Cholesky S = U^T U and U^-1 in LDS by one block (FP64).  A is n x n
-// column-major; on return its upper triangle holds U^-1 and the strict lower
-// triangle is zero.  info = k+1 if the k-th leading minor is not positive.
-__global__ void __launch_bounds__(1024) k_chol_inv(double* __restrict__ A, int n, int* __restrict__ info) {
+// Cholesky factor S = U^T U in LDS by one block (FP64, n*n + n doubles of
+// LDS).  A is n x n column-major; on return A holds U (strict lower triangle
+// zero) and Ut holds U row-major (= U^T column-major) for k_trinv_cols.
+// info = k+1 if the k-th leading minor is not positive.
+__global__ void __launch_bounds__(1024) k_chol_lds(double* __restrict__ A, double* __restrict__ Ut, int n,
+                                                   int* __restrict__ info) {
   extern __shared__ double smem_d[];
   double* L = smem_d;            // n*n
   double* tmp = smem_d + n * n;  // n
@@ -192,7 +194,7 @@ __global__ void __launch_bounds__(1024) k_chol_inv(double* __restrict__ A, int n
   __syncthreads();
   int bad = 0;
   // right-looking factorisation with deferred row scaling: after step k the
-  // trailing block holds S_kk... minus sum_{p<k} a_pi a_pj / a_pp
+  // trailing upper triangle holds S_ij - sum_{p<=k} a_pi a_pj / a_pp
   for (int k = 0; k < n; ++k) {
     const double akk = L[k + (size_t)k * n];
     if (!(akk > 0.0)) {
@@ -211,27 +213,70 @@ __global__ void __launch_bounds__(1024) k_chol_inv(double* __restrict__ A, int n
     if (tid == 0) *info = bad;
     return;
   }
-  for (int k = tid; k < n; k += nt) tmp[k] = sqrt(L[k + (size_t)k * n]);
+  for (int k = tid; k < n; k += nt) tmp[k] = 1.0 / sqrt(L[k + (size_t)k * n]);
   __syncthreads();
   for (int e = tid; e < n * n; e += nt) {
     const int i = e % n, j = e / n;
-    L[e] = (i <= j) ? L[e] / tmp[i] : 0.0;
+    const double u = (i <= j) ? L[e] * tmp[i] : 0.0;
+    A[e] = u;
+    Ut[(size_t)i * n + j] = u;
   }
+}
+
+// Ut = U row-major from a factor left in the upper triangle of A by potrf.
+__global__ void k_upper_to_rows(const double* __restrict__ A, double* __restrict__ Ut, int n) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < (size_t)n * n; e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n), j = (int)(e % n);  // Ut[i][j] = U(i, j)
+    Ut[e] = i <= j ? A[i + (size_t)j * n] : 0.0;
+  }
+}
+
+__device__ __forceinline__ double wave_allsum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// X = U^-1 one column per wave: back substitution U x = e_j,
+// x_i = -(sum_{k=i+1..j} U_ik x_k) / U_ii, with the rows of U read from Ut
+// (contiguous) one step ahead of the reduction that needs them.  X may be the
+// storage of U (column-major): column j is written only by block j and U is
+// read from Ut.  Q = ceil(n / 64).
+template <int Q>
+__global__ void __launch_bounds__(64) k_trinv_cols(const double* __restrict__ Ut, int n, double* __restrict__ X) {
+  __shared__ double x[64 * Q];
+  const int j = blockIdx.x, lane = threadIdx.x;
+  for (int k = lane; k < 64 * Q; k += 64) x[k] = 0.0;
   __syncthreads();
-  // U^-1 column by column: X(i,j) = -(sum_{k=i}^{j-1} X(i,k) U(k,j)) / U(j,j)
-  for (int j = 0; j < n; ++j) {
-    const double ujj = L[j + (size_t)j * n];
-    for (int i = tid; i < j; i += nt) {
-      double s = 0.0;
-      for (int k = i; k < j; ++k) s += L[i + (size_t)k * n] * L[k + (size_t)j * n];
-      tmp[i] = -s / ujj;
+  if (lane == 0) x[j] = 1.0 / Ut[(size_t)j * n + j];
+  __syncthreads();
+  double ucur[Q], unext[Q];
+  double dcur = 0.0, dnext = 0.0;
+  // row i = j - 1: entries k = i + 1 + lane + 64 q <= j
+  auto load_row = [&](int i, double* u, double& d) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int k = i + 1 + lane + 64 * q;
+      u[q] = k <= j ? Ut[(size_t)i * n + k] : 0.0;
     }
+    d = Ut[(size_t)i * n + i];
+  };
+  if (j > 0) load_row(j - 1, ucur, dcur);
+  for (int i = j - 1; i >= 0; --i) {
+    if (i > 0) load_row(i - 1, unext, dnext);
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int k = i + 1 + lane + 64 * q;
+      if (k <= j) s += ucur[q] * x[k];
+    }
+    s = wave_allsum(s);
+    if (lane == 0) x[i] = -s / dcur;
     __syncthreads();
-    for (int i = tid; i < j; i += nt) L[i + (size_t)j * n] = tmp[i];
-    if (tid == 0) L[j + (size_t)j * n] = 1.0 / ujj;
-    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) ucur[q] = unext[q];
+    dcur = dnext;
   }
-  for (int e = tid; e < n * n; e += nt) A[e] = L[e];
+  for (int k = lane; k < n; k += 64) X[k + (size_t)j * n] = x[k];
 }
 
 }  // namespace amdk

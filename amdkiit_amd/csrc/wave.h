@@ -253,14 +253,7 @@ __global__ void k_pcg_update(V* __restrict__ C0, V* __restrict__ H, const V* __r
   }
 }
 
-// upper-triangular clean-up after trtri (rocsolver leaves the strict lower
-// triangle untouched) and occupation masking of the Lagrange matrix.
-__global__ void k_triu(double* A, int n, int lda) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n * n; i += gridDim.x * blockDim.x) {
-    const int r = i % n, c = i / n;
-    if (r > c) A[(size_t)c * lda + r] = 0.0;
-  }
-}
+// occupation masking of the Lagrange matrix.
 __global__ void k_mask_occ(double* A, int n, const double* __restrict__ occ) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n * n; i += gridDim.x * blockDim.x) {
     const int r = i % n, c = i / n;

@@ -204,14 +204,16 @@ def main():
         hp2 = sum(p2[k][0] / p2[k][1] for k in HPSI_KERNELS if k in p2 and p2[k][1])
         hb2 = hpsi_bytes(s2.pw, s2.nbands, "mixed")
         cx.close()
-        c64, d64, e64b, _, _ = run_gpu(s2, "fp64", st2, a.warmup, c02, profile_steps=1)
+        c64, d64, e64b, p64, _ = run_gpu(s2, "fp64", st2, a.warmup, c02, profile_steps=1)
         c64.close()
         secondary = {"workload": CONFIGS["h2o_64_pbe"], "steps": st2,
                      "mixed_steps_per_s": st2 / d2, "mixed_ms_per_step": d2 / st2 * 1e3,
                      "fp64_steps_per_s": st2 / d64, "mixed_speedup": d64 / d2,
                      "hpsi_ms": hp2, "hpsi_GBps": hb2 / (hp2 * 1e-3) / 1e9,
                      "hpsi_frac_hbm": hb2 / (hp2 * 1e-3) / 1e9 / HBM_PEAK,
-                     "etot_diff_per_atom": abs(e2["etot"] - e64b["etot"]) / s2.nat}
+                     "etot_diff_per_atom": abs(e2["etot"] - e64b["etot"]) / s2.nat,
+                     "mixed_ms_by_region": {k: v[0] / 2 for k, v in sorted(p2.items(), key=lambda x: -x[1][0])},
+                     "fp64_ms_by_region": {k: v[0] for k, v in sorted(p64.items(), key=lambda x: -x[1][0])}}
         log(f"[bench] 64 H2O: mixed {secondary['mixed_ms_per_step']:.2f} ms/step, fp64 speedup "
             f"{secondary['mixed_speedup']:.2f}x, H|psi> {secondary['hpsi_GBps']:.0f} GB/s")
 
