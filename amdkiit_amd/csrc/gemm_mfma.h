@@ -49,18 +49,34 @@ __device__ __forceinline__ void acc_flush(dbl4& hi, flt4& lo) {
   lo = (flt4){0.f, 0.f, 0.f, 0.f};
 }
 
-constexpr int OVM_T = 64, OVM_KT = 32;
+constexpr int OVM_T = 64, OVM_KT = 64;
 
-// Block tile 64 x 64 of the result, 4 waves of 32 x 32 (2 x 2 MFMA tiles).
+// MI355X hands consecutive workgroups round-robin to its 8 XCDs, each with
+// its own L2.  Physical block p runs logical block xcd_block(p, per) so that
+// every XCD works through one contiguous range of `per` logical blocks: the
+// tiles that share operand rows are then resident in the same L2.  Launch
+// 8 * per blocks; logical ids >= the real count exit at once.
+__device__ __forceinline__ int xcd_block(int p, int per) { return (p & 7) * per + (p >> 3); }
+
+// Block tile 64 x 64 of the result, 4 waves of 32 x 32 (2 x 2 MFMA tiles);
+// logical block = (split, n-tile, m-tile), m fastest.  The next 64-deep K
+// tile is loaded into registers (double2, 512 B per row) while the current
+// one is multiplied from LDS.
 template <typename TC>
 __global__ void __launch_bounds__(256) k_ovl_mfma(const double* __restrict__ X, int ldx, int M,
                                                   const double* __restrict__ Y, int ldy, int N, int K,
-                                                  int kchunk, double* __restrict__ part) {
+                                                  int kchunk, int nsplit, int per, double* __restrict__ part) {
   typedef typename Acc4<TC>::type AccT;
   constexpr bool F64 = sizeof(TC) == 8;
+  constexpr int KH = OVM_KT / 2;                      // double2 per row and tile
+  constexpr int LD = (OVM_T * KH) / 256;              // double2 loads per thread and operand
   __shared__ TC xs[OVM_T][OVM_KT + 1];
   __shared__ TC ys[OVM_T][OVM_KT + 1];
-  const int m0 = blockIdx.x * OVM_T, n0 = blockIdx.y * OVM_T, split = blockIdx.z;
+  const int tm = (M + OVM_T - 1) / OVM_T, tn = (N + OVM_T - 1) / OVM_T;
+  const int L = xcd_block(blockIdx.x, per);
+  if (L >= tm * tn * nsplit) return;
+  const int split = L / (tm * tn), t = L % (tm * tn);
+  const int m0 = (t % tm) * OVM_T, n0 = (t / tm) * OVM_T;
   const int kb = split * kchunk, ke = min(K, kb + kchunk);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
@@ -74,13 +90,28 @@ __global__ void __launch_bounds__(256) k_ovl_mfma(const double* __restrict__ X, 
       acc[a][b] = (dbl4){0.0, 0.0, 0.0, 0.0};
       tac[a][b] = (AccT){0, 0, 0, 0};
     }
+  double2 px[LD], py[LD];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int it = 0; it < LD; ++it) {
+      const int e = threadIdx.x + 256 * it, r = e / KH, k = k0 + 2 * (e % KH);
+      // K and every chunk start are even, so a pair never straddles ke
+      px[it] = (m0 + r < M && k < ke) ? *(const double2*)(X + (size_t)(m0 + r) * ldx + k) : make_double2(0.0, 0.0);
+      py[it] = (n0 + r < N && k < ke) ? *(const double2*)(Y + (size_t)(n0 + r) * ldy + k) : make_double2(0.0, 0.0);
+    }
+  };
+  load(kb);
   for (int k0 = kb; k0 < ke; k0 += OVM_KT) {
-    for (int e = threadIdx.x; e < OVM_T * OVM_KT; e += 256) {
-      const int r = e / OVM_KT, kk = e % OVM_KT, k = k0 + kk;
-      xs[r][kk] = (TC)((m0 + r < M && k < ke) ? X[(size_t)(m0 + r) * ldx + k] : 0.0);
-      ys[r][kk] = (TC)((n0 + r < N && k < ke) ? Y[(size_t)(n0 + r) * ldy + k] : 0.0);
+#pragma unroll
+    for (int it = 0; it < LD; ++it) {
+      const int e = threadIdx.x + 256 * it, r = e / KH, c = 2 * (e % KH);
+      xs[r][c] = (TC)px[it].x;
+      xs[r][c + 1] = (TC)px[it].y;
+      ys[r][c] = (TC)py[it].x;
+      ys[r][c + 1] = (TC)py[it].y;
     }
     __syncthreads();
+    if (k0 + OVM_KT < ke) load(k0 + OVM_KT);
 #pragma unroll
     for (int ks = 0; ks < OVM_KT; ks += 4) {
       const int kk = ks + lk;
@@ -97,12 +128,14 @@ __global__ void __launch_bounds__(256) k_ovl_mfma(const double* __restrict__ X, 
         tac[1][0] = mfma16(a1, b0, tac[1][0]);
         tac[1][1] = mfma16(a1, b1, tac[1][1]);
       }
-    }
-    if constexpr (!F64) {
+      if constexpr (!F64) {
+        if ((ks & 31) == 28) {  // FP64 sum of every 32-deep FP32 tile
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+          for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc_flush(acc[a][b], tac[a][b]);
+            for (int b = 0; b < 2; ++b) acc_flush(acc[a][b], tac[a][b]);
+        }
+      }
     }
     __syncthreads();
   }
@@ -119,21 +152,31 @@ __global__ void __launch_bounds__(256) k_ovl_mfma(const double* __restrict__ X, 
       }
 }
 
-constexpr int UPM_MC = 32, UPM_NB = 128;
+constexpr int UPM_MC = 32, UPM_NB = 128, UPM_K = 64;
 
-// Block: 64 reals of k (4 waves x 16) x 128 columns n (8 MFMA tiles per wave).
+// Block: 64 reals of k (4 waves x 16) x 128 columns n (8 MFMA tiles per
+// wave); logical block = (k-block, n-block), n fastest, so the n-blocks that
+// read the same rows of X share an XCD's L2.  Chunks of 32 m: X (32 x 64)
+// and Q (32 x 128) are staged in LDS, the next chunk prefetched into
+// registers while the current one is multiplied.
 template <typename TC>
 __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X, int ldx, int M,
                                                   const double* __restrict__ Q, int ldq,
                                                   double* __restrict__ Y, int ldy, int N, int K,
-                                                  double alpha, double beta) {
+                                                  double alpha, double beta, int per) {
   typedef typename Acc4<TC>::type AccT;
   constexpr bool F64 = sizeof(TC) == 8;
+  constexpr int XL = (UPM_MC * UPM_K / 2) / 256;   // double2 of X per thread
+  constexpr int QL = (UPM_MC * UPM_NB / 2) / 256;  // double2 of Q per thread
   __shared__ TC qs[UPM_MC][UPM_NB + 1];
+  __shared__ TC xs[UPM_MC][UPM_K + 1];
+  const int nnb = (N + UPM_NB - 1) / UPM_NB, nkb = (K + UPM_K - 1) / UPM_K;
+  const int L = xcd_block(blockIdx.x, per);
+  if (L >= nnb * nkb) return;
+  const int k0 = (L / nnb) * UPM_K, n0 = (L % nnb) * UPM_NB;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = lane & 15, lk = lane >> 4;
-  const int k = blockIdx.x * 64 + wave * 16 + li;
-  const int n0 = blockIdx.y * UPM_NB;
+  const int kl = wave * 16 + li, k = k0 + kl;
   dbl4 acc[8];
   AccT tac[8];
 #pragma unroll
@@ -141,16 +184,40 @@ __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X, 
     acc[t] = (dbl4){0.0, 0.0, 0.0, 0.0};
     tac[t] = (AccT){0, 0, 0, 0};
   }
+  double2 px[XL], pq[QL];
+  auto load = [&](int mb) {
+#pragma unroll
+    for (int it = 0; it < XL; ++it) {  // X rows mb.., 32 double2 per row
+      const int e = threadIdx.x + 256 * it, r = e / (UPM_K / 2), kk = k0 + 2 * (e % (UPM_K / 2));
+      px[it] = (mb + r < M && kk < K) ? *(const double2*)(X + (size_t)(mb + r) * ldx + kk) : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int it = 0; it < QL; ++it) {  // Q columns n0.., 16 double2 (m) per column
+      const int e = threadIdx.x + 256 * it, j = e / (UPM_MC / 2), i = 2 * (e % (UPM_MC / 2));
+      const bool ok = n0 + j < N;
+      const double* q = Q + (size_t)(n0 + j) * ldq + mb + i;
+      pq[it] = make_double2(ok && mb + i < M ? q[0] : 0.0, ok && mb + i + 1 < M ? q[1] : 0.0);
+    }
+  };
+  load(0);
   for (int mb = 0; mb < M; mb += UPM_MC) {
-    for (int e = threadIdx.x; e < UPM_MC * UPM_NB; e += 256) {
-      const int i = e % UPM_MC, j = e / UPM_MC;  // m fastest: contiguous in a column of Q
-      qs[i][j] = (TC)((mb + i < M && n0 + j < N) ? Q[(size_t)(n0 + j) * ldq + mb + i] : 0.0);
+#pragma unroll
+    for (int it = 0; it < XL; ++it) {
+      const int e = threadIdx.x + 256 * it, r = e / (UPM_K / 2), c = 2 * (e % (UPM_K / 2));
+      xs[r][c] = (TC)px[it].x;
+      xs[r][c + 1] = (TC)px[it].y;
+    }
+#pragma unroll
+    for (int it = 0; it < QL; ++it) {
+      const int e = threadIdx.x + 256 * it, j = e / (UPM_MC / 2), i = 2 * (e % (UPM_MC / 2));
+      qs[i][j] = (TC)pq[it].x;
+      qs[i + 1][j] = (TC)pq[it].y;
     }
     __syncthreads();
+    if (mb + UPM_MC < M) load(mb + UPM_MC);
     const int mend = min(UPM_MC, M - mb);
     for (int ms = 0; ms < mend; ms += 4) {
-      const int m = mb + ms + lk;
-      const TC b = (TC)((m < M && k < K) ? X[(size_t)m * ldx + k] : 0.0);
+      const TC b = xs[ms + lk][kl];
       if constexpr (F64) {
 #pragma unroll
         for (int t = 0; t < 8; ++t) acc[t] = mfma16(qs[ms + lk][16 * t + li], b, acc[t]);
