@@ -130,7 +130,6 @@ struct amdk_ctx {
   int nb_state = 0;
   std::vector<double> h_occ_state;
   void* rocblas = nullptr;    // rocblas_handle
-  int chol_lds_max = -1;      // max LDS bytes per workgroup (queried once)
   // allocations to free
   std::vector<void*> owned;
   std::string err;

@@ -181,17 +181,30 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
   double acc[A];
 #pragma unroll
   for (int j1 = 0; j1 < A; ++j1) acc[j1] = 0.0;
-  for (int pl = 0; pl < npairs; ++pl) {
+  // FP32 with <= 10 values per thread: the next pair's planes are loaded into
+  // registers while this one is transformed (wider forms would lose occupancy)
+  constexpr bool PF = sizeof(T) == 4 && TM <= 10;
+  V rn[B];
+  auto load_pair = [&](int pl, V* dst) {
     const V* col = W2 + (size_t)pl * g.nplane * n2 * n1 + (size_t)y * n1 + x;
+#pragma unroll
+    for (int k2 = 0; k2 < B; ++k2) {
+      const int kz = signed_freq(t + A * k2, N);
+      if (t < A && live && kz >= -g.R && kz <= g.R) dst[k2] = col[(size_t)(kz + g.R) * n2 * n1];
+      else { dst[k2].x = 0; dst[k2].y = 0; }
+    }
+  };
+  if (PF && npairs > 0) load_pair(0, rn);
+  for (int pl = 0; pl < npairs; ++pl) {
     V r[TM];
 #pragma unroll
     for (int k = 0; k < TM; ++k) { r[k].x = 0; r[k].y = 0; }
-    if (t < A && live) {
+    if constexpr (PF) {
 #pragma unroll
-      for (int k2 = 0; k2 < B; ++k2) {
-        const int kz = signed_freq(t + A * k2, N);
-        if (kz >= -g.R && kz <= g.R) r[k2] = col[(size_t)(kz + g.R) * n2 * n1];
-      }
+      for (int k2 = 0; k2 < B; ++k2) r[k2] = rn[k2];
+      if (pl + 1 < npairs) load_pair(pl + 1, rn);
+    } else {
+      load_pair(pl, r);
     }
     col_fft<V, A, B, +1>(r, t, lds, TX, c, tw);
     const double w1 = wpair[2 * pl], w2 = wpair[2 * pl + 1];

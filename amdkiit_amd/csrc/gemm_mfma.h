@@ -246,48 +246,74 @@ __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X, 
   }
 }
 
-// Cholesky factor S = U^T U in LDS by one block (FP64, n*n + n doubles of
-// LDS).  A is n x n column-major; on return A holds U (strict lower triangle
-// zero) and Ut holds U row-major (= U^T column-major) for k_trinv_cols.
-// info = k+1 if the k-th leading minor is not positive.
-__global__ void __launch_bounds__(1024) k_chol_lds(double* __restrict__ A, double* __restrict__ Ut, int n,
+// Cholesky factor S = U^T U by one block of 32 x 32 threads with the matrix
+// in registers: thread (ti, tj) owns S(ti + 32 a, tj + 32 b), a, b < NPT
+// (n <= 32 NPT).  Step k broadcasts row k through LDS and updates the
+// trailing upper triangle with deferred row scaling (a_ij -= a_ki a_kj / a_kk).
+// On return A holds U (strict lower triangle zero) and Ut holds U row-major
+// (= U^T column-major) for k_trinv_cols.  info = k+1 if the k-th leading
+// minor is not positive.
+template <int NPT>
+__global__ void __launch_bounds__(1024) k_chol_reg(double* __restrict__ A, double* __restrict__ Ut, int n,
                                                    int* __restrict__ info) {
-  extern __shared__ double smem_d[];
-  double* L = smem_d;            // n*n
-  double* tmp = smem_d + n * n;  // n
-  const int tid = threadIdx.x, nt = blockDim.x;
-  for (int e = tid; e < n * n; e += nt) L[e] = A[e];
-  if (tid == 0) *info = 0;
-  __syncthreads();
-  int bad = 0;
-  // right-looking factorisation with deferred row scaling: after step k the
-  // trailing upper triangle holds S_ij - sum_{p<=k} a_pi a_pj / a_pp
-  for (int k = 0; k < n; ++k) {
-    const double akk = L[k + (size_t)k * n];
-    if (!(akk > 0.0)) {
-      bad = k + 1;
-      break;
+  __shared__ double rowk[32 * NPT];
+  __shared__ double diag[32 * NPT];
+  const int ti = threadIdx.x & 31, tj = threadIdx.x >> 5;
+  double v[NPT][NPT];
+#pragma unroll
+  for (int a = 0; a < NPT; ++a)
+#pragma unroll
+    for (int b = 0; b < NPT; ++b) {
+      const int i = ti + 32 * a, j = tj + 32 * b;
+      v[a][b] = (i < n && j < n) ? A[i + (size_t)j * n] : 0.0;
     }
-    const double inv = 1.0 / akk;
-    const int m = n - k - 1;
-    for (int e = tid; e < m * m; e += nt) {
-      const int i = k + 1 + e % m, j = k + 1 + e / m;
-      if (i <= j) L[i + (size_t)j * n] -= L[k + (size_t)i * n] * L[k + (size_t)j * n] * inv;
+  if (threadIdx.x == 0) *info = 0;
+  for (int k = 0; k < n; ++k) {
+    if (ti == (k & 31)) {
+#pragma unroll
+      for (int a = 0; a < NPT; ++a)
+        if (a == (k >> 5)) {
+#pragma unroll
+          for (int b = 0; b < NPT; ++b) rowk[tj + 32 * b] = v[a][b];
+        }
     }
     __syncthreads();
+    const double akk = rowk[k];
+    if (!(akk > 0.0)) {  // uniform across the block
+      if (threadIdx.x == 0) *info = k + 1;
+      return;
+    }
+    const double inv = 1.0 / akk;
+    double ri[NPT], rj[NPT];
+#pragma unroll
+    for (int a = 0; a < NPT; ++a) {
+      ri[a] = rowk[ti + 32 * a] * inv;
+      rj[a] = rowk[tj + 32 * a];
+    }
+#pragma unroll
+    for (int a = 0; a < NPT; ++a)
+#pragma unroll
+      for (int b = 0; b < NPT; ++b) {
+        const int i = ti + 32 * a, j = tj + 32 * b;
+        if (i > k && i <= j && j < n) v[a][b] -= ri[a] * rj[b];
+      }
+    __syncthreads();
   }
-  if (bad) {
-    if (tid == 0) *info = bad;
-    return;
-  }
-  for (int k = tid; k < n; k += nt) tmp[k] = 1.0 / sqrt(L[k + (size_t)k * n]);
+#pragma unroll
+  for (int a = 0; a < NPT; ++a)
+    if (ti == tj && ti + 32 * a < n) diag[ti + 32 * a] = 1.0 / sqrt(v[a][a]);
   __syncthreads();
-  for (int e = tid; e < n * n; e += nt) {
-    const int i = e % n, j = e / n;
-    const double u = (i <= j) ? L[e] * tmp[i] : 0.0;
-    A[e] = u;
-    Ut[(size_t)i * n + j] = u;
-  }
+#pragma unroll
+  for (int a = 0; a < NPT; ++a)
+#pragma unroll
+    for (int b = 0; b < NPT; ++b) {
+      const int i = ti + 32 * a, j = tj + 32 * b;
+      if (i < n && j < n) {
+        const double u = i <= j ? v[a][b] * diag[i] : 0.0;
+        A[i + (size_t)j * n] = u;
+        Ut[(size_t)i * n + j] = u;
+      }
+    }
 }
 
 // Ut = U row-major from a factor left in the upper triangle of A by potrf.
@@ -303,47 +329,62 @@ __device__ __forceinline__ double wave_allsum(double v) {
   return v;
 }
 
-// X = U^-1 one column per wave: back substitution U x = e_j,
-// x_i = -(sum_{k=i+1..j} U_ik x_k) / U_ii, with the rows of U read from Ut
-// (contiguous) one step ahead of the reduction that needs them.  X may be the
-// storage of U (column-major): column j is written only by block j and U is
-// read from Ut.  Q = ceil(n / 64).
-template <int Q>
-__global__ void __launch_bounds__(64) k_trinv_cols(const double* __restrict__ Ut, int n, double* __restrict__ X) {
-  __shared__ double x[64 * Q];
-  const int j = blockIdx.x, lane = threadIdx.x;
-  for (int k = lane; k < 64 * Q; k += 64) x[k] = 0.0;
+// X = U^-1, one block of 256 threads per column j: back substitution
+// U x = e_j, x_i = -(sum_{k=i+1..j} U_ik x_k) / U_ii, in chunks of C rows
+// (descending).  Per chunk the rows of U (from Ut, row-major) are staged in
+// LDS, the part of each row's sum over already known x is formed by all
+// threads, and one wave finishes the C x C triangle with x held in lanes.
+// X may be the storage of U (column-major): column j is written only by
+// block j and U is read from Ut.  n <= NMAX.
+template <int NMAX, int C>
+__global__ void __launch_bounds__(256) k_trinv_cols(const double* __restrict__ Ut, int n, double* __restrict__ X) {
+  static_assert(C <= 64 && 256 % C == 0, "chunk rows");
+  __shared__ double x[NMAX];
+  __shared__ double uc[C][NMAX + 1];
+  __shared__ double part[C];
+  __shared__ double red[256];
+  const int j = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int k = tid; k < NMAX; k += 256) x[k] = 0.0;
   __syncthreads();
-  if (lane == 0) x[j] = 1.0 / Ut[(size_t)j * n + j];
+  if (tid == 0) x[j] = 1.0 / Ut[(size_t)j * n + j];
   __syncthreads();
-  double ucur[Q], unext[Q];
-  double dcur = 0.0, dnext = 0.0;
-  // row i = j - 1: entries k = i + 1 + lane + 64 q <= j
-  auto load_row = [&](int i, double* u, double& d) {
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int k = i + 1 + lane + 64 * q;
-      u[q] = k <= j ? Ut[(size_t)i * n + k] : 0.0;
+  for (int i1 = j; i1 > 0;) {
+    const int i0 = max(0, i1 - C), rows = i1 - i0, w = j - i0 + 1;  // columns i0..j
+    for (int e = tid; e < rows * w; e += 256) {
+      const int r = e / w, kk = e % w;
+      uc[r][kk] = Ut[(size_t)(i0 + r) * n + i0 + kk];
     }
-    d = Ut[(size_t)i * n + i];
-  };
-  if (j > 0) load_row(j - 1, ucur, dcur);
-  for (int i = j - 1; i >= 0; --i) {
-    if (i > 0) load_row(i - 1, unext, dnext);
-    double s = 0.0;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int k = i + 1 + lane + 64 * q;
-      if (k <= j) s += ucur[q] * x[k];
-    }
-    s = wave_allsum(s);
-    if (lane == 0) x[i] = -s / dcur;
     __syncthreads();
-#pragma unroll
-    for (int q = 0; q < Q; ++q) ucur[q] = unext[q];
-    dcur = dnext;
+    // part[r] = sum_{k = i1..j} U(i0 + r, k) x_k by 256 / C threads per row
+    {
+      constexpr int TPR = 256 / C;
+      const int r = tid / TPR, q = tid % TPR;
+      double s = 0.0;
+      if (r < rows)
+        for (int k = i1 + q; k <= j; k += TPR) s += uc[r][k - i0] * x[k];
+      red[tid] = s;
+      __syncthreads();
+      if (q == 0 && r < rows) {
+        double t = 0.0;
+        for (int u = 0; u < TPR; ++u) t += red[tid + u];
+        part[r] = t;
+      }
+      __syncthreads();
+    }
+    if (wave == 0) {
+      double xl = 0.0;  // x_{i0 + lane} once known
+      for (int r = rows - 1; r >= 0; --r) {
+        const double term = (lane > r && lane < rows) ? uc[r][lane] * xl : 0.0;
+        const double s = wave_allsum(term) + part[r];
+        const double xi = -s / uc[r][r];
+        if (lane == r) xl = xi;
+      }
+      if (lane < rows) x[i0 + lane] = xl;
+    }
+    __syncthreads();
+    i1 = i0;
   }
-  for (int k = lane; k < n; k += 64) X[k + (size_t)j * n] = x[k];
+  for (int k = tid; k < n; k += 256) X[k + (size_t)j * n] = x[k];
 }
 
 }  // namespace amdk

@@ -172,51 +172,22 @@ __global__ void k_dbeta(const V* __restrict__ beta, const int* __restrict__ mill
   }
 }
 
-// ---------------------------------------------------------------------------
-// PCG (cg.f90:63-145).  P = f^-1 K(G)^-1 C2, ||P||^2 reduced, then
-// d = P + gamma d_prev and C0 += step d.
+// Partial sums of sum_b w_b <P_b|P_b> for the preconditioned gradient
+// P = f^-1 K^-1 C2 (P itself is recomputed in k_pcg_update, not stored).
 template <typename V>
-__global__ void k_precon(const V* __restrict__ C2, V* __restrict__ P, int ldc, int npw, int nb,
-                         const double* __restrict__ vpp, const double* __restrict__ finv) {
-  const int b = blockIdx.y;
-  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ldc; g += gridDim.x * blockDim.x) {
-    const size_t i = (size_t)b * ldc + g;
-    V o;
-    if (g < npw) {
-      const double s = finv[b] * vpp[g];
-      o.x = s * (double)C2[i].x;
-      o.y = s * (double)C2[i].y;
-    } else {
-      o.x = 0;
-      o.y = 0;
-    }
-    P[i] = o;
-  }
-}
-
-// Fused: P = f^-1 K^-1 C2 (zero padding) and partial sums of sum_b w_b <P_b|P_b>.
-template <typename V>
-__global__ void __launch_bounds__(256) k_precon_norm(const V* __restrict__ C2, V* __restrict__ P, int ldc, int npw,
-                                                     int nb, const double* __restrict__ vpp,
-                                                     const double* __restrict__ finv, const double* __restrict__ w,
-                                                     double* __restrict__ part) {
+__global__ void __launch_bounds__(256) k_precon_norm(const V* __restrict__ C2, int ldc, int npw, int nb,
+                                                     const double* __restrict__ vpp, const double* __restrict__ finv,
+                                                     const double* __restrict__ w, double* __restrict__ part) {
   __shared__ double sh[4];
   double s = 0;
   const size_t tot = (size_t)nb * ldc;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
     const int b = (int)(i / ldc), g = (int)(i % ldc);
-    V o;
     if (g < npw) {
       const double f = finv[b] * vpp[g];
-      o.x = f * (double)C2[i].x;
-      o.y = f * (double)C2[i].y;
-      const double a2 = (double)o.x * o.x + (double)o.y * o.y;
-      s += w[b] * (g == 0 ? (double)o.x * o.x : 2.0 * a2);
-    } else {
-      o.x = 0;
-      o.y = 0;
+      const double px = f * (double)C2[i].x, py = f * (double)C2[i].y;
+      s += w[b] * (g == 0 ? px * px : 2.0 * (px * px + py * py));
     }
-    P[i] = o;
   }
   const double r = block_sum(s, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = r;
@@ -236,15 +207,27 @@ __global__ void k_pcg_scalar(double* st, int istep) {
   }
 }
 
-// d = P + gamma d ; C0 += step d   (d stored in H, P read from P)
+// d = P + gamma d ; C0 += step d  (d stored in H; P = f^-1 K^-1 C2 formed on
+// the fly, zero in the padding).  Grid: x over plane waves, y over bands.
 template <typename V>
-__global__ void k_pcg_update(V* __restrict__ C0, V* __restrict__ H, const V* __restrict__ P, size_t n,
+__global__ void k_pcg_update(V* __restrict__ C0, V* __restrict__ H, const V* __restrict__ C2, int ldc, int npw,
+                             const double* __restrict__ vpp, const double* __restrict__ finv,
                              const double* __restrict__ st, double step) {
   const double gamma = st[2];
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+  const int b = blockIdx.y;
+  const double fb = finv[b];
+  const size_t off = (size_t)b * ldc;
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ldc; g += gridDim.x * blockDim.x) {
+    const size_t i = off + g;
+    double px = 0.0, py = 0.0;
+    if (g < npw) {
+      const double f = fb * vpp[g];
+      px = f * (double)C2[i].x;
+      py = f * (double)C2[i].y;
+    }
     V d;
-    d.x = (double)P[i].x + gamma * (double)H[i].x;
-    d.y = (double)P[i].y + gamma * (double)H[i].y;
+    d.x = px + gamma * (double)H[i].x;
+    d.y = py + gamma * (double)H[i].y;
     H[i] = d;
     V c = C0[i];
     c.x = (double)c.x + step * (double)d.x;
