@@ -248,66 +248,72 @@ __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X, 
 
 // Cholesky factor S = U^T U by one block of 32 x 32 threads with the matrix
 // in registers: thread (ti, tj) owns S(ti + 32 a, tj + 32 b), a, b < NPT
-// (n <= 32 NPT).  Step k broadcasts row k through LDS and updates the
-// trailing upper triangle with deferred row scaling (a_ij -= a_ki a_kj / a_kk).
-// On return A holds U (strict lower triangle zero) and Ut holds U row-major
-// (= U^T column-major) for k_trinv_cols.  info = k+1 if the k-th leading
-// minor is not positive.
+// (n <= 32 NPT), upper triangle only.  Step k updates the trailing upper
+// triangle with deferred row scaling (a_ij -= a_ki a_kj / a_kk) from row k,
+// which its owners publish in a double-buffered LDS row at the end of step
+// k - 1 (one barrier per step); row blocks a that lie wholly above k are
+// skipped.  On return A holds U (strict lower triangle zero) and Ut holds U
+// row-major (= U^T column-major) for k_trinv_cols.  info = k+1 if the k-th
+// leading minor is not positive.
 template <int NPT>
 __global__ void __launch_bounds__(1024) k_chol_reg(double* __restrict__ A, double* __restrict__ Ut, int n,
                                                    int* __restrict__ info) {
-  __shared__ double rowk[32 * NPT];
-  __shared__ double diag[32 * NPT];
-  const int ti = threadIdx.x & 31, tj = threadIdx.x >> 5;
+  constexpr int T = 32, NM = T * NPT;
+  __shared__ double rowk[2][NM];
+  __shared__ double diag[NM];
+  const int ti = threadIdx.x % T, tj = threadIdx.x / T;
   double v[NPT][NPT];
 #pragma unroll
   for (int a = 0; a < NPT; ++a)
 #pragma unroll
     for (int b = 0; b < NPT; ++b) {
-      const int i = ti + 32 * a, j = tj + 32 * b;
-      v[a][b] = (i < n && j < n) ? A[i + (size_t)j * n] : 0.0;
+      const int i = ti + T * a, j = tj + T * b;
+      v[a][b] = (i <= j && j < n) ? A[i + (size_t)j * n] : 0.0;
     }
   if (threadIdx.x == 0) *info = 0;
-  for (int k = 0; k < n; ++k) {
-    if (ti == (k & 31)) {
+  auto publish = [&](int k, double* dst) {  // owners of row k write it
+    if (ti == k % T) {
 #pragma unroll
       for (int a = 0; a < NPT; ++a)
-        if (a == (k >> 5)) {
+        if (a == k / T) {
 #pragma unroll
-          for (int b = 0; b < NPT; ++b) rowk[tj + 32 * b] = v[a][b];
+          for (int b = 0; b < NPT; ++b) dst[tj + T * b] = v[a][b];
         }
     }
-    __syncthreads();
-    const double akk = rowk[k];
-    if (!(akk > 0.0)) {  // uniform across the block
+  };
+  publish(0, rowk[0]);
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    const double* rk = rowk[k & 1];
+    const double akk = rk[k];
+    if (!(akk > 0.0)) {  // the same value in every thread
       if (threadIdx.x == 0) *info = k + 1;
       return;
     }
     const double inv = 1.0 / akk;
-    double ri[NPT], rj[NPT];
 #pragma unroll
     for (int a = 0; a < NPT; ++a) {
-      ri[a] = rowk[ti + 32 * a] * inv;
-      rj[a] = rowk[tj + 32 * a];
-    }
-#pragma unroll
-    for (int a = 0; a < NPT; ++a)
+      if (T * a + T - 1 <= k) continue;  // rows of this block all done (uniform)
+      const int i = ti + T * a;
+      const double fi = rk[i] * inv;
 #pragma unroll
       for (int b = 0; b < NPT; ++b) {
-        const int i = ti + 32 * a, j = tj + 32 * b;
-        if (i > k && i <= j && j < n) v[a][b] -= ri[a] * rj[b];
+        const int j = tj + T * b;
+        if (i > k && i <= j && j < n) v[a][b] -= fi * rk[j];
       }
+    }
+    if (k + 1 < n) publish(k + 1, rowk[(k + 1) & 1]);
     __syncthreads();
   }
 #pragma unroll
   for (int a = 0; a < NPT; ++a)
-    if (ti == tj && ti + 32 * a < n) diag[ti + 32 * a] = 1.0 / sqrt(v[a][a]);
+    if (ti == tj && ti + T * a < n) diag[ti + T * a] = 1.0 / sqrt(v[a][a]);
   __syncthreads();
 #pragma unroll
   for (int a = 0; a < NPT; ++a)
 #pragma unroll
     for (int b = 0; b < NPT; ++b) {
-      const int i = ti + 32 * a, j = tj + 32 * b;
+      const int i = ti + T * a, j = tj + T * b;
       if (i < n && j < n) {
         const double u = i <= j ? v[a][b] * diag[i] : 0.0;
         A[i + (size_t)j * n] = u;
@@ -324,16 +330,18 @@ __global__ void k_upper_to_rows(const double* __restrict__ A, double* __restrict
   }
 }
 
-__device__ __forceinline__ double wave_allsum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+// value of v in lane l (l uniform across the wave)
+__device__ __forceinline__ double bcast_lane(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
 }
 
 // X = U^-1, one block of 256 threads per column j: back substitution
 // U x = e_j, x_i = -(sum_{k=i+1..j} U_ik x_k) / U_ii, in chunks of C rows
 // (descending).  Per chunk the rows of U (from Ut, row-major) are staged in
 // LDS, the part of each row's sum over already known x is formed by all
-// threads, and one wave finishes the C x C triangle with x held in lanes.
+// threads, and one wave finishes the C x C triangle, one row per lane.
 // X may be the storage of U (column-major): column j is written only by
 // block j and U is read from Ut.  n <= NMAX.
 template <int NMAX, int C>
@@ -372,12 +380,13 @@ __global__ void __launch_bounds__(256) k_trinv_cols(const double* __restrict__ U
       __syncthreads();
     }
     if (wave == 0) {
-      double xl = 0.0;  // x_{i0 + lane} once known
-      for (int r = rows - 1; r >= 0; --r) {
-        const double term = (lane > r && lane < rows) ? uc[r][lane] * xl : 0.0;
-        const double s = wave_allsum(term) + part[r];
-        const double xi = -s / uc[r][r];
-        if (lane == r) xl = xi;
+      // lane r keeps the running sum of row i0 + r; each x_l, once known in
+      // lane l, is broadcast with readlane and folded into the rows above
+      double acc = lane < rows ? part[lane] : 0.0, xl = 0.0;
+      for (int l = rows - 1; l >= 0; --l) {
+        if (lane == l) xl = -acc / uc[l][l];
+        const double xb = bcast_lane(xl, l);
+        if (lane < l) acc += uc[lane][l] * xb;
       }
       if (lane < rows) x[i0 + lane] = xl;
     }
