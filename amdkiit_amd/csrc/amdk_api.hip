@@ -194,9 +194,15 @@ static void build_sphere(amdk_ctx* c, const int* mill, int ncaller, Sphere& S) {
   upload(c, S.d_conj_caller, S.h_conj_caller.data(), sizeof(int) * ncaller);
 }
 
+// Row pitch of the W1/W2 work arrays: x rows padded to 16 elements (128 B in
+// FP32, 256 B in FP64) so that the x-tiles of the y and z passes start on
+// cache-line boundaries.
+static int row_pitch(int n1) { return (n1 + 15) / 16 * 16; }
+
 static PassGeom geom_of(const amdk_ctx* c, const Sphere& S) {
   PassGeom g;
   g.n1 = c->n1; g.n2 = c->n2; g.n3 = c->n3;
+  g.p1 = row_pitch(c->n1);
   g.nrow = S.nrow; g.nslot = S.nslot; g.nplane = S.nplane; g.R = S.R;
   g.rows = S.rows; g.slot_m2 = S.slot_m2; g.plane_off = S.plane_off;
   return g;
@@ -615,8 +621,8 @@ int amdk_ctx_create(const amdk_desc* d, amdk_ctx** out) {
     ctx->vr = dev_alloc(ctx, (ctx->prec == AMDK_PREC_FP64 ? sizeof(double) : sizeof(float)) * nr);
     // workspaces: pairs per chunk bounded by a 4 GiB W2 budget
     const Sphere& S = ctx->spw;
-    const size_t w2_pair = (size_t)S.nplane * d->n2 * d->n1 * cbytes(ctx);
-    const size_t w1_pair = (size_t)S.nslot * d->n1 * cbytes(ctx);
+    const size_t w2_pair = (size_t)S.nplane * d->n2 * row_pitch(d->n1) * cbytes(ctx);
+    const size_t w1_pair = (size_t)S.nslot * row_pitch(d->n1) * cbytes(ctx);
     const int npairs = (d->nbmax + 1) / 2;
     const size_t budget = (size_t)4 << 30;
     ctx->cap_pairs = (int)std::max<size_t>(1, std::min<size_t>(npairs, budget / (w2_pair + w1_pair)));

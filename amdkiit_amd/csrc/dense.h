@@ -36,7 +36,7 @@ __global__ void __launch_bounds__(256) k_z_fwd_dense(PassGeom g, const double* _
   double2* res = lds_fft<double2, -1>(b0, b1, tw, p3, TX, 1, TX, true);
   for (int idx = threadIdx.x; idx < g.nplane * TX; idx += blockDim.x) {
     const int p = idx / TX, tx = idx % TX, x = x0 + tx;
-    if (x < n1) W2[((size_t)p * n2 + y) * n1 + x] = res[wrap(p - g.R, n3) * TX + tx];
+    if (x < n1) W2[((size_t)p * n2 + y) * g.p1 + x] = res[wrap(p - g.R, n3) * TX + tx];
   }
 }
 
@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(256) k_z_inv_dense(PassGeom g, const double2* 
   __syncthreads();
   for (int idx = threadIdx.x; idx < g.nplane * TX; idx += blockDim.x) {
     const int p = idx / TX, tx = idx % TX, x = x0 + tx;
-    if (x < n1) b0[wrap(p - g.R, n3) * TX + tx] = W2[((size_t)p * n2 + y) * n1 + x];
+    if (x < n1) b0[wrap(p - g.R, n3) * TX + tx] = W2[((size_t)p * n2 + y) * g.p1 + x];
   }
   __syncthreads();
   double2* res = lds_fft<double2, +1>(b0, b1, tw, p3, TX, 1, TX, true);
@@ -89,10 +89,10 @@ __global__ void __launch_bounds__(256) k_rows_fwd_unpack(PassGeom g, const doubl
   load_tw(tw, twg, n1);
   for (int lr = 0; lr < nr; ++lr) {
     const RowInfo ri = g.rows[r0 + lr];
-    const double2* id = W1 + (size_t)ri.slot_d * n1;
+    const double2* id = W1 + (size_t)ri.slot_d * g.p1;
     for (int i = threadIdx.x; i < n1; i += blockDim.x) b0[(2 * lr) * n1 + i] = id[i];
     if (ri.slot_m >= 0) {
-      const double2* im = W1 + (size_t)ri.slot_m * n1;
+      const double2* im = W1 + (size_t)ri.slot_m * g.p1;
       for (int i = threadIdx.x; i < n1; i += blockDim.x) b0[(2 * lr + 1) * n1 + i] = im[i];
     } else {
       for (int i = threadIdx.x; i < n1; i += blockDim.x) b0[(2 * lr + 1) * n1 + i] = make_double2(0, 0);

@@ -20,6 +20,7 @@ namespace amdk {
 
 struct PassGeom {
   int n1, n2, n3;
+  int p1;  // row pitch (elements) of the W1/W2 work arrays, >= n1, 128-byte multiple
   int nrow, nslot, nplane, R;
   const RowInfo* rows;
   const int* slot_m2;
@@ -92,10 +93,10 @@ __global__ void __launch_bounds__(256) k_rows_inv(PassGeom g, const double2* __r
   V* res = lds_fft<V, +1>(b0, b1, tw, p1, 2 * nr, n1, 1, false);
   for (int lr = 0; lr < nr; ++lr) {
     const RowInfo ri = g.rows[r0 + lr];
-    V* od = W1 + ((size_t)pl * g.nslot + ri.slot_d) * n1;
+    V* od = W1 + ((size_t)pl * g.nslot + ri.slot_d) * g.p1;
     for (int i = threadIdx.x; i < n1; i += blockDim.x) od[i] = res[(2 * lr) * n1 + i];
     if (ri.slot_m >= 0) {
-      V* om = W1 + ((size_t)pl * g.nslot + ri.slot_m) * n1;
+      V* om = W1 + ((size_t)pl * g.nslot + ri.slot_m) * g.p1;
       for (int i = threadIdx.x; i < n1; i += blockDim.x) om[i] = res[(2 * lr + 1) * n1 + i];
     }
   }
@@ -124,14 +125,14 @@ __global__ void __launch_bounds__(256) k_cols_inv(PassGeom g, const typename cpl
     const int s = s0 + idx / TX;
     const int tx = idx % TX;
     const int x = x0 + tx;
-    if (x < n1) b0[wrap(g.slot_m2[s], n2) * TX + tx] = W1[((size_t)pl * g.nslot + s) * n1 + x];
+    if (x < n1) b0[wrap(g.slot_m2[s], n2) * TX + tx] = W1[((size_t)pl * g.nslot + s) * g.p1 + x];
   }
   __syncthreads();
   V* res = lds_fft<V, +1>(b0, b1, tw, p2, TX, 1, TX, true);
-  V* out = W2 + ((size_t)pl * g.nplane + plane) * (size_t)n2 * n1;
+  V* out = W2 + ((size_t)pl * g.nplane + plane) * (size_t)n2 * g.p1;
   for (int idx = threadIdx.x; idx < n2 * TX; idx += blockDim.x) {
     const int y = idx / TX, tx = idx % TX, x = x0 + tx;
-    if (x < n1) out[(size_t)y * n1 + x] = res[idx];
+    if (x < n1) out[(size_t)y * g.p1 + x] = res[idx];
   }
 }
 
@@ -151,11 +152,11 @@ __global__ void __launch_bounds__(256) k_cols_fwd(PassGeom g, const typename cpl
   const int plane = blockIdx.y;
   const int pl = blockIdx.z;
   load_tw(tw, twg, n2);
-  const V* in = W2 + ((size_t)pl * g.nplane + plane) * (size_t)n2 * n1;
+  const V* in = W2 + ((size_t)pl * g.nplane + plane) * (size_t)n2 * g.p1;
   for (int idx = threadIdx.x; idx < n2 * TX; idx += blockDim.x) {
     const int y = idx / TX, tx = idx % TX, x = x0 + tx;
     V v;
-    if (x < n1) v = in[(size_t)y * n1 + x]; else { v.x = 0; v.y = 0; }
+    if (x < n1) v = in[(size_t)y * g.p1 + x]; else { v.x = 0; v.y = 0; }
     b0[idx] = v;
   }
   __syncthreads();
@@ -165,7 +166,7 @@ __global__ void __launch_bounds__(256) k_cols_fwd(PassGeom g, const typename cpl
     const int s = s0 + idx / TX;
     const int tx = idx % TX;
     const int x = x0 + tx;
-    if (x < n1) W1[((size_t)pl * g.nslot + s) * n1 + x] = res[wrap(g.slot_m2[s], n2) * TX + tx];
+    if (x < n1) W1[((size_t)pl * g.nslot + s) * g.p1 + x] = res[wrap(g.slot_m2[s], n2) * TX + tx];
   }
 }
 
@@ -195,10 +196,10 @@ __global__ void __launch_bounds__(256) k_z_vpsi(PassGeom g, typename cplx_of<T>:
   for (int pl = pg0; pl < pg1; ++pl) {
     for (int i = threadIdx.x; i < n3 * TX; i += blockDim.x) { b0[i].x = 0; b0[i].y = 0; }
     __syncthreads();
-    V* col = W2 + (size_t)pl * g.nplane * n2 * n1 + (size_t)y * n1;
+    V* col = W2 + (size_t)pl * g.nplane * n2 * g.p1 + (size_t)y * g.p1;
     for (int idx = threadIdx.x; idx < g.nplane * TX; idx += blockDim.x) {
       const int p = idx / TX, tx = idx % TX, x = x0 + tx;
-      if (x < n1) b0[wrap(p - g.R, n3) * TX + tx] = col[(size_t)p * n2 * n1 + x];
+      if (x < n1) b0[wrap(p - g.R, n3) * TX + tx] = col[(size_t)p * n2 * g.p1 + x];
     }
     __syncthreads();
     V* r1 = lds_fft<V, +1>(b0, b1, tw, p3, TX, 1, TX, true);
@@ -212,7 +213,7 @@ __global__ void __launch_bounds__(256) k_z_vpsi(PassGeom g, typename cplx_of<T>:
     V* r2 = lds_fft<V, -1>(r1, other, tw, p3, TX, 1, TX, true);
     for (int idx = threadIdx.x; idx < g.nplane * TX; idx += blockDim.x) {
       const int p = idx / TX, tx = idx % TX, x = x0 + tx;
-      if (x < n1) col[(size_t)p * n2 * n1 + x] = r2[wrap(p - g.R, n3) * TX + tx];
+      if (x < n1) col[(size_t)p * n2 * g.p1 + x] = r2[wrap(p - g.R, n3) * TX + tx];
     }
     __syncthreads();
   }
@@ -242,10 +243,10 @@ __global__ void __launch_bounds__(256) k_z_rho(PassGeom g, const typename cplx_o
   for (int pl = 0; pl < npairs; ++pl) {
     for (int i = threadIdx.x; i < n3 * TX; i += blockDim.x) { b0[i].x = 0; b0[i].y = 0; }
     __syncthreads();
-    const V* col = W2 + (size_t)pl * g.nplane * n2 * n1 + (size_t)y * n1;
+    const V* col = W2 + (size_t)pl * g.nplane * n2 * g.p1 + (size_t)y * g.p1;
     for (int idx = threadIdx.x; idx < g.nplane * TX; idx += blockDim.x) {
       const int p = idx / TX, tx = idx % TX, x = x0 + tx;
-      if (x < n1) b0[wrap(p - g.R, n3) * TX + tx] = col[(size_t)p * n2 * n1 + x];
+      if (x < n1) b0[wrap(p - g.R, n3) * TX + tx] = col[(size_t)p * n2 * g.p1 + x];
     }
     __syncthreads();
     V* r = lds_fft<V, +1>(b0, b1, tw, p3, TX, 1, TX, true);
@@ -298,10 +299,10 @@ __global__ void __launch_bounds__(256) k_rows_fwd_hpsi(PassGeom g, const typenam
   load_tw(tw, twg, n1);
   for (int lr = 0; lr < nr; ++lr) {
     const RowInfo ri = g.rows[r0 + lr];
-    const V* id = W1 + ((size_t)pl * g.nslot + ri.slot_d) * n1;
+    const V* id = W1 + ((size_t)pl * g.nslot + ri.slot_d) * g.p1;
     for (int i = threadIdx.x; i < n1; i += blockDim.x) b0[(2 * lr) * n1 + i] = id[i];
     if (ri.slot_m >= 0) {
-      const V* im = W1 + ((size_t)pl * g.nslot + ri.slot_m) * n1;
+      const V* im = W1 + ((size_t)pl * g.nslot + ri.slot_m) * g.p1;
       for (int i = threadIdx.x; i < n1; i += blockDim.x) b0[(2 * lr + 1) * n1 + i] = im[i];
     } else {
       for (int i = threadIdx.x; i < n1; i += blockDim.x) { b0[(2 * lr + 1) * n1 + i].x = 0; b0[(2 * lr + 1) * n1 + i].y = 0; }

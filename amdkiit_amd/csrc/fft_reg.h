@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
   __syncthreads();
   const int pg0 = blockIdx.z * PG, pg1 = min(npairs, pg0 + PG);
   for (int pl = pg0; pl < pg1; ++pl) {
-    V* col = W2 + (size_t)pl * g.nplane * n2 * n1 + (size_t)y * n1 + x;
+    V* col = W2 + (size_t)pl * g.nplane * n2 * g.p1 + (size_t)y * g.p1 + x;
     V r[TM];
 #pragma unroll
     for (int k = 0; k < TM; ++k) { r[k].x = 0; r[k].y = 0; }
@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
 #pragma unroll
       for (int k2 = 0; k2 < B; ++k2) {
         const int kz = signed_freq(t + A * k2, N);
-        if (kz >= -g.R && kz <= g.R) r[k2] = col[(size_t)(kz + g.R) * n2 * n1];
+        if (kz >= -g.R && kz <= g.R) r[k2] = col[(size_t)(kz + g.R) * n2 * g.p1];
       }
     }
     col_fft<V, A, B, +1>(r, t, lds, TX, c, tw);
@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
 #pragma unroll
       for (int j1 = 0; j1 < B; ++j1) {
         const int kz = signed_freq(t + A * j1, N);
-        if (kz >= -g.R && kz <= g.R) col[(size_t)(kz + g.R) * n2 * n1] = r[j1];
+        if (kz >= -g.R && kz <= g.R) col[(size_t)(kz + g.R) * n2 * g.p1] = r[j1];
       }
     }
   }
@@ -186,11 +186,11 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
   constexpr bool PF = sizeof(T) == 4 && TM <= 10;
   V rn[B];
   auto load_pair = [&](int pl, V* dst) {
-    const V* col = W2 + (size_t)pl * g.nplane * n2 * n1 + (size_t)y * n1 + x;
+    const V* col = W2 + (size_t)pl * g.nplane * n2 * g.p1 + (size_t)y * g.p1 + x;
 #pragma unroll
     for (int k2 = 0; k2 < B; ++k2) {
       const int kz = signed_freq(t + A * k2, N);
-      if (t < A && live && kz >= -g.R && kz <= g.R) dst[k2] = col[(size_t)(kz + g.R) * n2 * n1];
+      if (t < A && live && kz >= -g.R && kz <= g.R) dst[k2] = col[(size_t)(kz + g.R) * n2 * g.p1];
       else { dst[k2].x = 0; dst[k2].y = 0; }
     }
   };
@@ -250,14 +250,14 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
 #pragma unroll
     for (int k2 = 0; k2 < B; ++k2) {
       const int ky = signed_freq(t + A * k2, N);
-      if (ky >= lo && ky <= hi) r[k2] = W1[((size_t)pl * g.nslot + s0 + ky - lo) * n1 + x];
+      if (ky >= lo && ky <= hi) r[k2] = W1[((size_t)pl * g.nslot + s0 + ky - lo) * g.p1 + x];
     }
   }
   col_fft<V, A, B, +1>(r, t, lds, TX, c, tw);
   if (t < B && live) {
-    V* out = W2 + ((size_t)pl * g.nplane + plane) * (size_t)n2 * n1 + x;
+    V* out = W2 + ((size_t)pl * g.nplane + plane) * (size_t)n2 * g.p1 + x;
 #pragma unroll
-    for (int j1 = 0; j1 < A; ++j1) out[(size_t)(t + B * j1) * n1] = r[j1];
+    for (int j1 = 0; j1 < A; ++j1) out[(size_t)(t + B * j1) * g.p1] = r[j1];
   }
 }
 
@@ -283,17 +283,17 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
   V r[TM];
 #pragma unroll
   for (int k = 0; k < TM; ++k) { r[k].x = 0; r[k].y = 0; }
-  const V* in = W2 + ((size_t)pl * g.nplane + plane) * (size_t)n2 * n1 + x;
+  const V* in = W2 + ((size_t)pl * g.nplane + plane) * (size_t)n2 * g.p1 + x;
   if (t < A && live) {
 #pragma unroll
-    for (int k2 = 0; k2 < B; ++k2) r[k2] = in[(size_t)(t + A * k2) * n1];
+    for (int k2 = 0; k2 < B; ++k2) r[k2] = in[(size_t)(t + A * k2) * g.p1];
   }
   col_fft<V, A, B, -1>(r, t, lds, TX, c, tw);
   if (t < B && live) {
 #pragma unroll
     for (int j1 = 0; j1 < A; ++j1) {
       const int ky = signed_freq(t + B * j1, N);
-      if (ky >= lo && ky <= hi) W1[((size_t)pl * g.nslot + s0 + ky - lo) * n1 + x] = r[j1];
+      if (ky >= lo && ky <= hi) W1[((size_t)pl * g.nslot + s0 + ky - lo) * g.p1 + x] = r[j1];
     }
   }
 }
@@ -352,7 +352,7 @@ __global__ void __launch_bounds__(2 * HR * (A > B ? A : B))
   col_fft<V, A, B, +1>(r, t, lds, 1, fr * ST, tw);
   if (live && t < B && !(zero_row && mirror)) {
     const int slot = mirror ? ri.slot_m : ri.slot_d;
-    V* out = W1 + ((size_t)pl * g.nslot + slot) * g.n1;
+    V* out = W1 + ((size_t)pl * g.nslot + slot) * g.p1;
 #pragma unroll
     for (int j1 = 0; j1 < A; ++j1) out[t + B * j1] = r[j1];
   }
@@ -386,7 +386,7 @@ __global__ void __launch_bounds__(2 * HR * (A > B ? A : B))
 #pragma unroll
   for (int k = 0; k < TM; ++k) { r[k].x = 0; r[k].y = 0; }
   if (live && t < A && !(zero_row && mirror)) {
-    const V* in = W1 + ((size_t)pl * g.nslot + (mirror ? ri.slot_m : ri.slot_d)) * g.n1;
+    const V* in = W1 + ((size_t)pl * g.nslot + (mirror ? ri.slot_m : ri.slot_d)) * g.p1;
 #pragma unroll
     for (int k2 = 0; k2 < B; ++k2) r[k2] = in[t + A * k2];
   }
@@ -462,7 +462,7 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
 #pragma unroll
     for (int j1 = 0; j1 < A; ++j1) {
       const int kz = signed_freq(t + B * j1, N);
-      if (kz >= -g.R && kz <= g.R) W2[((size_t)(kz + g.R) * n2 + y) * n1 + x] = r[j1];
+      if (kz >= -g.R && kz <= g.R) W2[((size_t)(kz + g.R) * n2 + y) * g.p1 + x] = r[j1];
     }
   }
 }
@@ -490,7 +490,7 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
 #pragma unroll
     for (int k2 = 0; k2 < B; ++k2) {
       const int kz = signed_freq(t + A * k2, N);
-      if (kz >= -g.R && kz <= g.R) r[k2] = W2[((size_t)(kz + g.R) * n2 + y) * n1 + x];
+      if (kz >= -g.R && kz <= g.R) r[k2] = W2[((size_t)(kz + g.R) * n2 + y) * g.p1 + x];
     }
   }
   col_fft<double2, A, B, +1>(r, t, lds, TX, c, tw);
@@ -528,7 +528,7 @@ __global__ void __launch_bounds__(2 * HR * (A > B ? A : B))
 #pragma unroll
   for (int k = 0; k < TM; ++k) r[k] = make_double2(0.0, 0.0);
   if (live && t < A && !(zero_row && mirror)) {
-    const double2* in = W1 + (size_t)(mirror ? ri.slot_m : ri.slot_d) * g.n1;
+    const double2* in = W1 + (size_t)(mirror ? ri.slot_m : ri.slot_d) * g.p1;
 #pragma unroll
     for (int k2 = 0; k2 < B; ++k2) r[k2] = in[t + A * k2];
   }

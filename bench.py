@@ -34,6 +34,24 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hpsi_h2o32_pbe_mixed_pmc_traffic.md")
+
+
+def pmc_traffic(config, precision):
+    """HBM bytes per launch measured with rocprofv3 PMC counters (FETCH_SIZE,
+    WRITE_SIZE, calibrated against a plain copy) in a separate profiling run
+    of this benchmark; read from the committed summary.  None for workloads
+    that have no such measurement."""
+    if config != "h2o_32_pbe" or precision != "mixed" or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    text = open(TRAFFIC_PROFILE).read()
+    i = text.find("```json")
+    if i < 0:
+        return None
+    j = text.find("```", i + 7)
+    return json.loads(text[i + 7:j])
+
+
 def hpsi_bytes(pw, nb, prec):
     """Algorithmic bytes of one local H|psi> application (all bands):
     (nb/2) * (36 N_r + 48 N_pw) in FP32, doubled in FP64."""
@@ -178,6 +196,11 @@ def main():
             per_kernel[k] = {"avg_ms": avg, "alg_bytes": kb[k], "GBps": kb[k] / (avg * 1e-3) / 1e9}
     step_breakdown = {k: v[0] / 3 for k, v in sorted(prof.items(), key=lambda x: -x[1][0])}
     dominant = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"])
+    traffic = pmc_traffic(a.config, a.precision)
+    if traffic:
+        for k in per_kernel:
+            tb = traffic["per_launch_bytes"].get("k_" + k + "_r")
+            per_kernel[k]["hbm_bytes_pmc"] = tb
 
     fp64 = None
     ctx.close()
@@ -242,7 +265,9 @@ def main():
                  "launches": list(HPSI_KERNELS)},
         "roofline": {"bound": "hbm", "kernel": "hpsi_local (" + "+".join(HPSI_KERNELS) + ")",
                      "achieved": hp_gbs, "peak": HBM_PEAK, "unit": "GB/s", "frac": hp_gbs / HBM_PEAK,
-                     "traffic": None, "dominant_launch": dominant, "per_launch": per_kernel},
+                     "traffic": traffic["hpsi_local_bytes"] if traffic else None,
+                     "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
+                     "dominant_launch": dominant, "per_launch": per_kernel},
         "step_ms_by_region": step_breakdown,
         "energy": {"etot": e["etot"], "gemax": e["gemax"], "nelec": e["nelec"], "after_steps": it},
         "fp64_gpu": fp64,
