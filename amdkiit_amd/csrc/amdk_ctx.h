@@ -113,7 +113,8 @@ struct amdk_ctx {
   double* d_gpart = nullptr;  // split-K partials
   size_t gpart_cap = 0;
   double* d_S = nullptr;      // nbmax x nbmax
-  double* d_Ut = nullptr;     // nbmax x nbmax: Cholesky factor, row-major
+  double* d_Ut = nullptr;     // nbmax x nbmax: Cholesky scratch (factor rows, block products)
+  double* d_X = nullptr;      // nbmax x nbmax: U^-1 of the orthonormalisation
   double* d_nl = nullptr;     // P x nbmax
   double* d_Q = nullptr;      // max(P, nbmax) x nbmax
   double* d_finv = nullptr;   // 1/f for PCG
@@ -129,7 +130,6 @@ struct amdk_ctx {
   void* Ct = nullptr;
   int nb_state = 0;
   std::vector<double> h_occ_state;
-  void* rocblas = nullptr;    // rocblas_handle
   // allocations to free
   std::vector<void*> owned;
   std::string err;

@@ -252,12 +252,13 @@ __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X, 
 // triangle with deferred row scaling (a_ij -= a_ki a_kj / a_kk) from row k,
 // which its owners publish in a double-buffered LDS row at the end of step
 // k - 1 (one barrier per step); row blocks a that lie wholly above k are
-// skipped.  On return A holds U (strict lower triangle zero) and Ut holds U
-// row-major (= U^T column-major) for k_trinv_cols.  info = k+1 if the k-th
-// leading minor is not positive.
+// skipped.  A has leading dimension lda; on return its n x n block holds U
+// (strict lower triangle zero) and Ut (n x n, contiguous) holds U row-major
+// (= U^T column-major) for k_trinv_cols.  A non-positive pivot k sets
+// info = info_base + k + 1 (info is never cleared here).
 template <int NPT>
-__global__ void __launch_bounds__(1024) k_chol_reg(double* __restrict__ A, double* __restrict__ Ut, int n,
-                                                   int* __restrict__ info) {
+__global__ void __launch_bounds__(1024) k_chol_reg(double* __restrict__ A, int lda, double* __restrict__ Ut, int n,
+                                                   int* __restrict__ info, int info_base) {
   constexpr int T = 32, NM = T * NPT;
   __shared__ double rowk[2][NM];
   __shared__ double diag[NM];
@@ -268,9 +269,8 @@ __global__ void __launch_bounds__(1024) k_chol_reg(double* __restrict__ A, doubl
 #pragma unroll
     for (int b = 0; b < NPT; ++b) {
       const int i = ti + T * a, j = tj + T * b;
-      v[a][b] = (i <= j && j < n) ? A[i + (size_t)j * n] : 0.0;
+      v[a][b] = (i <= j && j < n) ? A[i + (size_t)j * lda] : 0.0;
     }
-  if (threadIdx.x == 0) *info = 0;
   auto publish = [&](int k, double* dst) {  // owners of row k write it
     if (ti == k % T) {
 #pragma unroll
@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(1024) k_chol_reg(double* __restrict__ A, doubl
     const double* rk = rowk[k & 1];
     const double akk = rk[k];
     if (!(akk > 0.0)) {  // the same value in every thread
-      if (threadIdx.x == 0) *info = k + 1;
+      if (threadIdx.x == 0) *info = info_base + k + 1;
       return;
     }
     const double inv = 1.0 / akk;
@@ -316,7 +316,7 @@ __global__ void __launch_bounds__(1024) k_chol_reg(double* __restrict__ A, doubl
       const int i = ti + T * a, j = tj + T * b;
       if (i < n && j < n) {
         const double u = i <= j ? v[a][b] * diag[i] : 0.0;
-        A[i + (size_t)j * n] = u;
+        A[i + (size_t)j * lda] = u;
         Ut[(size_t)i * n + j] = u;
       }
     }
@@ -345,7 +345,7 @@ __device__ __forceinline__ double bcast_lane(double v, int l) {
 // X may be the storage of U (column-major): column j is written only by
 // block j and U is read from Ut.  n <= NMAX.
 template <int NMAX, int C>
-__global__ void __launch_bounds__(256) k_trinv_cols(const double* __restrict__ Ut, int n, double* __restrict__ X) {
+__global__ void __launch_bounds__(256) k_trinv_cols(const double* __restrict__ Ut, int n, double* __restrict__ X, int ldx) {
   static_assert(C <= 64 && 256 % C == 0, "chunk rows");
   __shared__ double x[NMAX];
   __shared__ double uc[C][NMAX + 1];
@@ -393,7 +393,48 @@ __global__ void __launch_bounds__(256) k_trinv_cols(const double* __restrict__ U
     __syncthreads();
     i1 = i0;
   }
-  for (int k = tid; k < n; k += 256) X[k + (size_t)j * n] = x[k];
+  for (int k = tid; k < n; k += 256) X[k + (size_t)j * ldx] = x[k];
+}
+
+// C = alpha op(A) op(B) + beta C for the small band-block products of the
+// blocked Cholesky (FP64, 16 x 16 output tiles, K in LDS chunks of 16);
+// op(X) = X or X^T (column-major storage).  beta == 0 ignores C's contents.
+__global__ void __launch_bounds__(256) k_gemm_small(int ta, int tb, int M, int N, int K, double alpha,
+                                                    const double* __restrict__ A, int lda,
+                                                    const double* __restrict__ B, int ldb, double beta,
+                                                    double* __restrict__ C, int ldc) {
+  __shared__ double as[16][17], bs[16][17];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int i = blockIdx.x * 16 + tx, j = blockIdx.y * 16 + ty;
+  double acc = 0.0;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    {  // as[r][kk] = op(A)(blockIdx.x*16 + r, k0 + kk)
+      const int r = tx, kk = ty, ii = blockIdx.x * 16 + r, k = k0 + kk;
+      as[r][kk] = (ii < M && k < K) ? (ta ? A[k + (size_t)ii * lda] : A[ii + (size_t)k * lda]) : 0.0;
+    }
+    {  // bs[kk][c] = op(B)(k0 + kk, blockIdx.y*16 + c)
+      const int kk = tx, cc = ty, jj = blockIdx.y * 16 + cc, k = k0 + kk;
+      bs[kk][cc] = (jj < N && k < K) ? (tb ? B[jj + (size_t)k * ldb] : B[k + (size_t)jj * ldb]) : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) acc += as[tx][kk] * bs[kk][ty];
+    __syncthreads();
+  }
+  if (i < M && j < N) {
+    double* c = C + i + (size_t)j * ldc;
+    *c = alpha * acc + (beta != 0.0 ? beta * *c : 0.0);
+  }
+}
+
+// X(0:m, 0:n) = 0 (column-major, leading dimension ldx)
+__global__ void k_zero_block(double* __restrict__ X, int ldx, int m, int n) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < (size_t)m * n; e += (size_t)gridDim.x * blockDim.x)
+    X[(e % m) + (e / m) * (size_t)ldx] = 0.0;
+}
+
+__global__ void k_set_int(int* __restrict__ p, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
 }  // namespace amdk

@@ -117,6 +117,33 @@ def run_gpu(sysobj, prec, steps, warmup, c0, profile_steps=3):
     return ctx, dt, e, prof, it
 
 
+MFMA_PEAK = {"f64": 78.6, "f32": 157.3}  # TFLOP/s, MI355X dense
+
+
+def gemm_table(prof, pw, nb, nproj, prec, nsteps):
+    """Achieved TFLOP/s of the band-space GEMMs of one SCF step (real form,
+    K = 2 N_pw reals; updates run over the padded leading dimension)."""
+    ldc = (pw.ngpw + 63) // 64 * 64
+    f32 = "f32" if prec == "mixed" else "f64"
+    shapes = {  # region: (M, N, K, arithmetic)
+        "fnl": (nproj, nb, 2 * pw.ngpw, f32),
+        "fnonloc": (nproj, nb, 2 * ldc, f32),
+        "lagrange_ovl": (nb, nb, 2 * pw.ngpw, f32),
+        "lagrange_upd": (nb, nb, 2 * ldc, f32),
+        "ortho_ovl": (nb, nb, 2 * pw.ngpw, "f64"),
+        "ortho_trmm": (nb, nb, 2 * ldc, "f64"),
+    }
+    out = {}
+    for k, (M, N, K, ar) in shapes.items():
+        if k not in prof or not prof[k][1] or M == 0:
+            continue
+        ms = prof[k][0] / prof[k][1]
+        tf = 2.0 * M * N * K / (ms * 1e-3) / 1e12
+        out[k] = {"M": M, "N": N, "K": K, "arith": ar, "avg_ms": ms, "TFLOPs": tf,
+                  "peak": MFMA_PEAK[ar], "frac": tf / MFMA_PEAK[ar]}
+    return out
+
+
 def cpu_baseline(s, c0, seconds_hint):
     """FP64 numpy/scipy restatement of the same SCF step on the host cores."""
     from oracle.ks_oracle import KSOracle
@@ -196,6 +223,7 @@ def main():
             per_kernel[k] = {"avg_ms": avg, "alg_bytes": kb[k], "GBps": kb[k] / (avg * 1e-3) / 1e9}
     step_breakdown = {k: v[0] / 3 for k, v in sorted(prof.items(), key=lambda x: -x[1][0])}
     dominant = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"])
+    gemms = gemm_table(prof, s.pw, s.nbands, ctx.nproj, a.precision, 3)
     traffic = pmc_traffic(a.config, a.precision)
     if traffic:
         for k in per_kernel:
@@ -226,6 +254,7 @@ def main():
         cx, d2, e2, p2, _ = run_gpu(s2, "mixed", st2, a.warmup, c02, profile_steps=2)
         hp2 = sum(p2[k][0] / p2[k][1] for k in HPSI_KERNELS if k in p2 and p2[k][1])
         hb2 = hpsi_bytes(s2.pw, s2.nbands, "mixed")
+        g2 = gemm_table(p2, s2.pw, s2.nbands, cx.nproj, "mixed", 2)
         cx.close()
         c64, d64, e64b, p64, _ = run_gpu(s2, "fp64", st2, a.warmup, c02, profile_steps=1)
         c64.close()
@@ -236,6 +265,7 @@ def main():
                      "hpsi_frac_hbm": hb2 / (hp2 * 1e-3) / 1e9 / HBM_PEAK,
                      "etot_diff_per_atom": abs(e2["etot"] - e64b["etot"]) / s2.nat,
                      "mixed_ms_by_region": {k: v[0] / 2 for k, v in sorted(p2.items(), key=lambda x: -x[1][0])},
+                     "mixed_gemm_mfma": g2,
                      "fp64_ms_by_region": {k: v[0] for k, v in sorted(p64.items(), key=lambda x: -x[1][0])}}
         log(f"[bench] 64 H2O: mixed {secondary['mixed_ms_per_step']:.2f} ms/step, fp64 speedup "
             f"{secondary['mixed_speedup']:.2f}x, H|psi> {secondary['hpsi_GBps']:.0f} GB/s")
@@ -269,6 +299,7 @@ def main():
                      "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
                      "dominant_launch": dominant, "per_launch": per_kernel},
         "step_ms_by_region": step_breakdown,
+        "gemm_mfma": gemms,
         "energy": {"etot": e["etot"], "gemax": e["gemax"], "nelec": e["nelec"], "after_steps": it},
         "fp64_gpu": fp64,
         "secondary_64h2o": secondary,
