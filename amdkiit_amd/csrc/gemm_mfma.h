@@ -159,11 +159,17 @@ constexpr int UPM_MC = 32, UPM_NB = 128, UPM_K = 64;
 // read the same rows of X share an XCD's L2.  Chunks of 32 m: X (32 x 64)
 // and Q (32 x 128) are staged in LDS, the next chunk prefetched into
 // registers while the current one is multiplied.
+// A second operand pair (X2: M2 rows, Q2) may follow the first: the sum runs
+// over the stacked rows (part 1 padded with zero rows to a whole 32-row
+// chunk), with Q1 scaled by a1 and Q2 by a2, so that two updates of Y share
+// one pass over it.
 template <typename TC>
-__global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X, int ldx, int M,
-                                                  const double* __restrict__ Q, int ldq,
+__global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X1, int ldx, int M1,
+                                                  const double* __restrict__ Q1, int ldq1, double a1,
+                                                  const double* __restrict__ X2, int M2,
+                                                  const double* __restrict__ Q2, int ldq2, double a2,
                                                   double* __restrict__ Y, int ldy, int N, int K,
-                                                  double alpha, double beta, int per) {
+                                                  double beta, int per) {
   typedef typename Acc4<TC>::type AccT;
   constexpr bool F64 = sizeof(TC) == 8;
   constexpr int XL = (UPM_MC * UPM_K / 2) / 256;   // double2 of X per thread
@@ -184,19 +190,28 @@ __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X, 
     acc[t] = (dbl4){0.0, 0.0, 0.0, 0.0};
     tac[t] = (AccT){0, 0, 0, 0};
   }
+  // stacked rows: part 1 padded to whole 32-row chunks, then part 2, so each
+  // chunk comes from one part (uniform per block)
+  const int M1p = (M1 + UPM_MC - 1) / UPM_MC * UPM_MC;
+  const int M = M1p + M2;
   double2 px[XL], pq[QL];
+  double qa = a1;  // scale of the chunk being staged
   auto load = [&](int mb) {
+    const bool two = mb >= M1p;
+    const double* X = two ? X2 : X1;
+    const double* Q = two ? Q2 : Q1;
+    const int ldq = two ? ldq2 : ldq1, Ms = two ? M2 : M1, m0 = two ? mb - M1p : mb;
 #pragma unroll
-    for (int it = 0; it < XL; ++it) {  // X rows mb.., 32 double2 per row
+    for (int it = 0; it < XL; ++it) {  // X rows m0.., 32 double2 per row
       const int e = threadIdx.x + 256 * it, r = e / (UPM_K / 2), kk = k0 + 2 * (e % (UPM_K / 2));
-      px[it] = (mb + r < M && kk < K) ? *(const double2*)(X + (size_t)(mb + r) * ldx + kk) : make_double2(0.0, 0.0);
+      px[it] = (m0 + r < Ms && kk < K) ? *(const double2*)(X + (size_t)(m0 + r) * ldx + kk) : make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int it = 0; it < QL; ++it) {  // Q columns n0.., 16 double2 (m) per column
       const int e = threadIdx.x + 256 * it, j = e / (UPM_MC / 2), i = 2 * (e % (UPM_MC / 2));
       const bool ok = n0 + j < N;
-      const double* q = Q + (size_t)(n0 + j) * ldq + mb + i;
-      pq[it] = make_double2(ok && mb + i < M ? q[0] : 0.0, ok && mb + i + 1 < M ? q[1] : 0.0);
+      const double* q = Q + (size_t)(n0 + j) * ldq + m0 + i;
+      pq[it] = make_double2(ok && m0 + i < Ms ? q[0] : 0.0, ok && m0 + i + 1 < Ms ? q[1] : 0.0);
     }
   };
   load(0);
@@ -207,11 +222,12 @@ __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X, 
       xs[r][c] = (TC)px[it].x;
       xs[r][c + 1] = (TC)px[it].y;
     }
+    qa = mb >= M1p ? a2 : a1;
 #pragma unroll
     for (int it = 0; it < QL; ++it) {
       const int e = threadIdx.x + 256 * it, j = e / (UPM_MC / 2), i = 2 * (e % (UPM_MC / 2));
-      qs[i][j] = (TC)pq[it].x;
-      qs[i + 1][j] = (TC)pq[it].y;
+      qs[i][j] = (TC)(qa * pq[it].x);
+      qs[i + 1][j] = (TC)(qa * pq[it].y);
     }
     __syncthreads();
     if (mb + UPM_MC < M) load(mb + UPM_MC);
@@ -240,7 +256,7 @@ __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X, 
         const int n = n0 + 16 * t + acc_row<TC>(lk, v);
         if (n < N) {
           double* y = Y + (size_t)n * ldy + k;
-          *y = (beta != 0.0 ? beta * *y : 0.0) + alpha * acc[t][v];
+          *y = (beta != 0.0 ? beta * *y : 0.0) + acc[t][v];
         }
       }
   }
