@@ -340,13 +340,15 @@ __global__ void __launch_bounds__(256) k_ekin(const V* __restrict__ C, int ldc, 
                                               const double* __restrict__ occ, double* __restrict__ part) {
   __shared__ double sh[4];
   double s = 0;
-  const size_t tot = (size_t)nb * npw;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
-    const int b = (int)(i / npw), g = (int)(i % npw);
-    double gx, gy, gz;
-    gvec_of(mill, g, rc, gx, gy, gz);
-    const V c = C[(size_t)b * ldc + g];
-    s += occ[b] * (gx * gx + gy * gy + gz * gz) * ((double)c.x * c.x + (double)c.y * c.y);
+  const int nch = (npw + 255) / 256, tot = nb * nch;  // (band, 256-wide chunk of G) per block step
+  for (int ch = blockIdx.x; ch < tot; ch += gridDim.x) {
+    const int b = ch / nch, g = (ch - b * nch) * 256 + threadIdx.x;
+    if (g < npw) {
+      double gx, gy, gz;
+      gvec_of(mill, g, rc, gx, gy, gz);
+      const V c = C[(size_t)b * ldc + g];
+      s += occ[b] * (gx * gx + gy * gy + gz * gz) * ((double)c.x * c.x + (double)c.y * c.y);
+    }
   }
   const double r = block_sum(s, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = r;

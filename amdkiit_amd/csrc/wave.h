@@ -48,23 +48,27 @@ __global__ void k_red_final(const double* __restrict__ part, int nq, double* __r
   }
 }
 
+// Band reductions walk (band, 256-wide chunk of G) pairs block by block, so
+// the band index comes from one scalar division per chunk.
 // sum_b w_b * DOTP(X_b, X_b) (full-sphere norm from the half sphere, G=0 once)
 // and max |X_b(G)| over b < nmax: part[0][blk] = sum, part[1][blk] = max.
 template <typename V>
 __global__ void __launch_bounds__(256) k_norms(const V* __restrict__ X, int ldx, int npw, int nb,
                                                const double* __restrict__ w, double* __restrict__ part) {
   __shared__ double sh[4];
-  double s = 0, mx = 0;
-  const size_t tot = (size_t)nb * npw;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
-    const int b = (int)(i / npw), g = (int)(i % npw);
-    const V x = X[(size_t)b * ldx + g];
-    const double a2 = (double)x.x * x.x + (double)x.y * x.y;
-    s += w[b] * (g == 0 ? (double)x.x * x.x : 2.0 * a2);
-    mx = fmax(mx, sqrt(a2));
+  double s = 0, mx2 = 0;
+  const int nch = (npw + 255) / 256, tot = nb * nch;
+  for (int c = blockIdx.x; c < tot; c += gridDim.x) {
+    const int b = c / nch, g = (c - b * nch) * 256 + threadIdx.x;
+    if (g < npw) {
+      const V x = X[(size_t)b * ldx + g];
+      const double a2 = (double)x.x * x.x + (double)x.y * x.y;
+      s += w[b] * (g == 0 ? (double)x.x * x.x : 2.0 * a2);
+      mx2 = fmax(mx2, a2);
+    }
   }
   const double rs = block_sum(s, sh);
-  const double rm = block_max(mx, sh);
+  const double rm = block_max(sqrt(mx2), sh);
   if (threadIdx.x == 0) {
     part[blockIdx.x] = rs;
     part[RED_BLOCKS + blockIdx.x] = rm;
@@ -180,10 +184,11 @@ __global__ void __launch_bounds__(256) k_precon_norm(const V* __restrict__ C2, i
                                                      const double* __restrict__ w, double* __restrict__ part) {
   __shared__ double sh[4];
   double s = 0;
-  const size_t tot = (size_t)nb * ldc;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
-    const int b = (int)(i / ldc), g = (int)(i % ldc);
+  const int nch = (npw + 255) / 256, tot = nb * nch;
+  for (int c = blockIdx.x; c < tot; c += gridDim.x) {
+    const int b = c / nch, g = (c - b * nch) * 256 + threadIdx.x;
     if (g < npw) {
+      const size_t i = (size_t)b * ldc + g;
       const double f = finv[b] * vpp[g];
       const double px = f * (double)C2[i].x, py = f * (double)C2[i].y;
       s += w[b] * (g == 0 ? px * px : 2.0 * (px * px + py * py));
