@@ -497,7 +497,7 @@ static void run_vpsi(amdk_ctx* c, const void* C0, void* C2, int nb) {
 }
 
 template <typename T>
-static void renormalise_density(amdk_ctx* c, const void* C0, int nb, double* rho);
+static void renormalise_density(amdk_ctx* c, const void* C0, int nb, double* rho, bool norms_ready);
 
 // Band weights of the density on the device: d_wpair[b] = f_b / Omega (pairs
 // read two consecutive entries), d_wdens[b] = f_b.  Set once per occupation.
@@ -511,7 +511,7 @@ static void prepare_density_weights(amdk_ctx* c, const double* occ, int nb) {
 
 // rho(r) for bands [0, nb) with the weights of prepare_density_weights
 template <typename T>
-static void run_density(amdk_ctx* c, const void* C0, int nb, double* rho) {
+static void run_density(amdk_ctx* c, const void* C0, int nb, double* rho, bool norms_ready = false) {
   const Sphere& S = c->spw;
   const int npairs = (nb + 1) / 2;
   for (int p0 = 0; p0 < npairs; p0 += c->cap_pairs) {
@@ -520,7 +520,7 @@ static void run_density(amdk_ctx* c, const void* C0, int nb, double* rho) {
     launch_cols<T>(c, S, np, true);
     launch_z_rho<T>(c, S, np, rho, p0 > 0, c->d_wpair + 2 * p0);
   }
-  if (sizeof(T) == 4) renormalise_density<T>(c, C0, nb, rho);
+  if (sizeof(T) == 4) renormalise_density<T>(c, C0, nb, rho, norms_ready);
 }
 
 // ---------------------------------------------------------------------------

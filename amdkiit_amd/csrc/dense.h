@@ -333,25 +333,35 @@ __global__ void __launch_bounds__(256) k_grid_sum(const double* __restrict__ x, 
   if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
 
-// E_kin partial: sum_b f_b sum_G |G|^2 |c|^2 (half sphere, total_energy.f90:38-61)
+
+// E_kin and the density norm of the bands in one pass over C0:
+// part4 = sum_b f_b sum_G |G|^2 |c|^2 over the half sphere (total_energy.f90:38-61),
+// part10 = sum_b f_b DOTP(c_b, c_b).
 template <typename V>
-__global__ void __launch_bounds__(256) k_ekin(const V* __restrict__ C, int ldc, int npw, int nb,
-                                              const int* __restrict__ mill, Recip rc,
-                                              const double* __restrict__ occ, double* __restrict__ part) {
+__global__ void __launch_bounds__(256) k_c0_sums(const V* __restrict__ C, int ldc, int npw, int nb,
+                                                 const int* __restrict__ mill, Recip rc,
+                                                 const double* __restrict__ occ, double* __restrict__ part4,
+                                                 double* __restrict__ part10) {
   __shared__ double sh[4];
-  double s = 0;
-  const int nch = (npw + 255) / 256, tot = nb * nch;  // (band, 256-wide chunk of G) per block step
+  double s = 0, q = 0;
+  const int nch = (npw + 255) / 256, tot = nb * nch;
   for (int ch = blockIdx.x; ch < tot; ch += gridDim.x) {
     const int b = ch / nch, g = (ch - b * nch) * 256 + threadIdx.x;
     if (g < npw) {
       double gx, gy, gz;
       gvec_of(mill, g, rc, gx, gy, gz);
       const V c = C[(size_t)b * ldc + g];
-      s += occ[b] * (gx * gx + gy * gy + gz * gz) * ((double)c.x * c.x + (double)c.y * c.y);
+      const double a2 = (double)c.x * c.x + (double)c.y * c.y;
+      s += occ[b] * (gx * gx + gy * gy + gz * gz) * a2;
+      q += occ[b] * (g == 0 ? (double)c.x * c.x : 2.0 * a2);
     }
   }
-  const double r = block_sum(s, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = r;
+  const double r4 = block_sum(s, sh);
+  const double r10 = block_sum(q, sh);
+  if (threadIdx.x == 0) {
+    part4[blockIdx.x] = r4;
+    part10[blockIdx.x] = r10;
+  }
 }
 
 // Local/Hartree force POTFOR (nuclear_grad.f90:4-50), one block per atom:

@@ -75,6 +75,38 @@ __global__ void __launch_bounds__(256) k_norms(const V* __restrict__ X, int ldx,
   }
 }
 
+// The gradient's three band reductions in one pass over C2 (bands b < nocc,
+// weights w_b): part5 = sum_b w_b DOTP(C2_b, C2_b) (CNORM), part6 = max |C2|
+// (GEMAX), part7 = sum_b w_b DOTP(P_b, P_b) for P = f_b^-1 K(G)^-1 C2 (PCG).
+template <typename V>
+__global__ void __launch_bounds__(256) k_grad_norms(const V* __restrict__ X, int ldx, int npw, int nocc,
+                                                    const double* __restrict__ vpp, const double* __restrict__ finv,
+                                                    const double* __restrict__ w, double* __restrict__ part5,
+                                                    double* __restrict__ part6, double* __restrict__ part7) {
+  __shared__ double sh[4];
+  double s = 0, p = 0, mx2 = 0;
+  const int nch = (npw + 255) / 256, tot = nocc * nch;
+  for (int c = blockIdx.x; c < tot; c += gridDim.x) {
+    const int b = c / nch, g = (c - b * nch) * 256 + threadIdx.x;
+    if (g < npw) {
+      const V x = X[(size_t)b * ldx + g];
+      const double a2 = (double)x.x * x.x + (double)x.y * x.y;
+      const double f = finv[b] * vpp[g];
+      s += w[b] * (g == 0 ? (double)x.x * x.x : 2.0 * a2);
+      p += w[b] * f * f * (g == 0 ? (double)x.x * x.x : 2.0 * a2);
+      mx2 = fmax(mx2, a2);
+    }
+  }
+  const double rs = block_sum(s, sh);
+  const double rp = block_sum(p, sh);
+  const double rm = block_max(sqrt(mx2), sh);
+  if (threadIdx.x == 0) {
+    part5[blockIdx.x] = rs;
+    part6[blockIdx.x] = rm;
+    part7[blockIdx.x] = rp;
+  }
+}
+
 template <typename V>
 __global__ void k_zclean(V* __restrict__ C, int ldc, int nb) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -176,28 +208,10 @@ __global__ void k_dbeta(const V* __restrict__ beta, const int* __restrict__ mill
   }
 }
 
-// Partial sums of sum_b w_b <P_b|P_b> for the preconditioned gradient
-// P = f^-1 K^-1 C2 (P itself is recomputed in k_pcg_update, not stored).
-template <typename V>
-__global__ void __launch_bounds__(256) k_precon_norm(const V* __restrict__ C2, int ldc, int npw, int nb,
-                                                     const double* __restrict__ vpp, const double* __restrict__ finv,
-                                                     const double* __restrict__ w, double* __restrict__ part) {
-  __shared__ double sh[4];
-  double s = 0;
-  const int nch = (npw + 255) / 256, tot = nb * nch;
-  for (int c = blockIdx.x; c < tot; c += gridDim.x) {
-    const int b = c / nch, g = (c - b * nch) * 256 + threadIdx.x;
-    if (g < npw) {
-      const size_t i = (size_t)b * ldc + g;
-      const double f = finv[b] * vpp[g];
-      const double px = f * (double)C2[i].x, py = f * (double)C2[i].y;
-      s += w[b] * (g == 0 ? px * px : 2.0 * (px * px + py * py));
-    }
-  }
-  const double r = block_sum(s, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = r;
-}
 
+// ---------------------------------------------------------------------------
+// PCG (cg.f90:63-145): P = f^-1 K(G)^-1 C2 (its norm from k_grad_norms, P
+// itself formed inside k_pcg_update), d = P + gamma d_prev, C0 += step d.
 // scalars: st[0] = ||P||^2 of this step (input), st[1] = previous ||P||^2,
 // st[2] = gamma out.  istep is 1-based within one optimisation.
 __global__ void k_pcg_scalar(double* st, int istep) {
