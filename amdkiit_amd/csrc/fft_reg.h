@@ -104,6 +104,19 @@ __device__ __forceinline__ void col_fft(V* r, int t, V* lds, int stride, int cof
 
 __device__ __forceinline__ int signed_freq(int p, int n) { return 2 * p <= n ? p : p - n; }
 
+// Element offset, in a pencil of planes or rows spaced `ps` apart, of the
+// frequency signed_freq(t + A k, N) relative to the first stored one (-lo),
+// or -1 when the frequency lies outside [lo, hi].  Written so that the
+// per-element work is adds and compares on `base` = (t - lo) * ps, hoisted by
+// the caller, instead of 64-bit index products.
+template <int A, int N>
+__device__ __forceinline__ int pencil_off(int t, int k, int lo, int hi, int base, int ps) {
+  const int z = t + A * k;
+  const bool wrap = 2 * z > N;
+  const int f = wrap ? z - N : z;
+  return (f >= lo && f <= hi) ? base + (A * k) * ps - (wrap ? N * ps : 0) : -1;
+}
+
 // ---------------------------------------------------------------------------
 // z pass of H|psi>: planes |m3| <= R -> z-FFT exp(+i) -> * v(r) -> z-FFT exp(-i)
 template <typename T, int A, int B, int TX>
@@ -129,16 +142,18 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
   }
   __syncthreads();
   const int pg0 = blockIdx.z * PG, pg1 = min(npairs, pg0 + PG);
+  const int ps = n2 * g.p1, base = (t + g.R) * ps;  // plane stride; plane kz sits at kz + R
+  const bool act = t < A && live;
   for (int pl = pg0; pl < pg1; ++pl) {
-    V* col = W2 + (size_t)pl * g.nplane * n2 * g.p1 + (size_t)y * g.p1 + x;
+    V* col = W2 + (size_t)pl * g.nplane * ps + (size_t)y * g.p1 + x;
     V r[TM];
 #pragma unroll
     for (int k = 0; k < TM; ++k) { r[k].x = 0; r[k].y = 0; }
-    if (t < A && live) {
+    if (act) {
 #pragma unroll
       for (int k2 = 0; k2 < B; ++k2) {
-        const int kz = signed_freq(t + A * k2, N);
-        if (kz >= -g.R && kz <= g.R) r[k2] = col[(size_t)(kz + g.R) * n2 * g.p1];
+        const int o = pencil_off<A, N>(t, k2, -g.R, g.R, base, ps);
+        if (o >= 0) r[k2] = col[o];
       }
     }
     col_fft<V, A, B, +1>(r, t, lds, TX, c, tw);
@@ -150,11 +165,11 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
       }
     }
     col_fft<V, B, A, -1>(r, t, lds, TX, c, tw);
-    if (t < A && live) {
+    if (act) {
 #pragma unroll
       for (int j1 = 0; j1 < B; ++j1) {
-        const int kz = signed_freq(t + A * j1, N);
-        if (kz >= -g.R && kz <= g.R) col[(size_t)(kz + g.R) * n2 * g.p1] = r[j1];
+        const int o = pencil_off<A, N>(t, j1, -g.R, g.R, base, ps);
+        if (o >= 0) col[o] = r[j1];
       }
     }
   }
@@ -185,12 +200,14 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
   // registers while this one is transformed (wider forms would lose occupancy)
   constexpr bool PF = sizeof(T) == 4 && TM <= 10;
   V rn[B];
+  const int ps = n2 * g.p1, base = (t + g.R) * ps;  // plane stride; plane kz sits at kz + R
+  const bool act = t < A && live;
   auto load_pair = [&](int pl, V* dst) {
-    const V* col = W2 + (size_t)pl * g.nplane * n2 * g.p1 + (size_t)y * g.p1 + x;
+    const V* col = W2 + (size_t)pl * g.nplane * ps + (size_t)y * g.p1 + x;
 #pragma unroll
     for (int k2 = 0; k2 < B; ++k2) {
-      const int kz = signed_freq(t + A * k2, N);
-      if (t < A && live && kz >= -g.R && kz <= g.R) dst[k2] = col[(size_t)(kz + g.R) * n2 * g.p1];
+      const int o = act ? pencil_off<A, N>(t, k2, -g.R, g.R, base, ps) : -1;
+      if (o >= 0) dst[k2] = col[o];
       else { dst[k2].x = 0; dst[k2].y = 0; }
     }
   };
@@ -247,17 +264,19 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
 #pragma unroll
   for (int k = 0; k < TM; ++k) { r[k].x = 0; r[k].y = 0; }
   if (t < A && live) {
+    const V* rows = W1 + ((size_t)pl * g.nslot + s0) * g.p1 + x;  // row of m2 = lo
+    const int base = (t - lo) * g.p1;
 #pragma unroll
     for (int k2 = 0; k2 < B; ++k2) {
-      const int ky = signed_freq(t + A * k2, N);
-      if (ky >= lo && ky <= hi) r[k2] = W1[((size_t)pl * g.nslot + s0 + ky - lo) * g.p1 + x];
+      const int o = pencil_off<A, N>(t, k2, lo, hi, base, g.p1);
+      if (o >= 0) r[k2] = rows[o];
     }
   }
   col_fft<V, A, B, +1>(r, t, lds, TX, c, tw);
   if (t < B && live) {
-    V* out = W2 + ((size_t)pl * g.nplane + plane) * (size_t)n2 * g.p1 + x;
+    V* out = W2 + ((size_t)pl * g.nplane + plane) * (size_t)n2 * g.p1 + (size_t)t * g.p1 + x;
 #pragma unroll
-    for (int j1 = 0; j1 < A; ++j1) out[(size_t)(t + B * j1) * g.p1] = r[j1];
+    for (int j1 = 0; j1 < A; ++j1) out[(B * j1) * g.p1] = r[j1];
   }
 }
 
@@ -283,17 +302,19 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
   V r[TM];
 #pragma unroll
   for (int k = 0; k < TM; ++k) { r[k].x = 0; r[k].y = 0; }
-  const V* in = W2 + ((size_t)pl * g.nplane + plane) * (size_t)n2 * g.p1 + x;
+  const V* in = W2 + ((size_t)pl * g.nplane + plane) * (size_t)n2 * g.p1 + (size_t)t * g.p1 + x;
   if (t < A && live) {
 #pragma unroll
-    for (int k2 = 0; k2 < B; ++k2) r[k2] = in[(size_t)(t + A * k2) * g.p1];
+    for (int k2 = 0; k2 < B; ++k2) r[k2] = in[(A * k2) * g.p1];
   }
   col_fft<V, A, B, -1>(r, t, lds, TX, c, tw);
   if (t < B && live) {
+    V* rows = W1 + ((size_t)pl * g.nslot + s0) * g.p1 + x;  // row of m2 = lo
+    const int base = (t - lo) * g.p1;
 #pragma unroll
     for (int j1 = 0; j1 < A; ++j1) {
-      const int ky = signed_freq(t + B * j1, N);
-      if (ky >= lo && ky <= hi) W1[((size_t)pl * g.nslot + s0 + ky - lo) * g.p1 + x] = r[j1];
+      const int o = pencil_off<B, N>(t, j1, lo, hi, base, g.p1);
+      if (o >= 0) rows[o] = r[j1];
     }
   }
 }
