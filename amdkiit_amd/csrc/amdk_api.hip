@@ -619,12 +619,18 @@ int amdk_ctx_create(const amdk_desc* d, amdk_ctx** out) {
     }
     const size_t nr = (size_t)d->n1 * d->n2 * d->n3;
     ctx->vr = dev_alloc(ctx, (ctx->prec == AMDK_PREC_FP64 ? sizeof(double) : sizeof(float)) * nr);
-    // workspaces: pairs per chunk bounded by a 4 GiB W2 budget
+    // FFT work arrays for up to cap_pairs band pairs at once (W1 + W2 within
+    // 24 GiB of the 288 GiB of HBM3E; AMDK_PAIR_BUDGET_MB overrides it, e.g.
+    // to exercise the chunked path at small sizes)
     const Sphere& S = ctx->spw;
     const size_t w2_pair = (size_t)S.nplane * d->n2 * row_pitch(d->n1) * cbytes(ctx);
     const size_t w1_pair = (size_t)S.nslot * row_pitch(d->n1) * cbytes(ctx);
     const int npairs = (d->nbmax + 1) / 2;
-    const size_t budget = (size_t)4 << 30;
+    size_t budget = (size_t)24 << 30;
+    if (const char* b = std::getenv("AMDK_PAIR_BUDGET_MB")) {
+      const long mb = std::atol(b);
+      if (mb > 0) budget = (size_t)mb << 20;
+    }
     ctx->cap_pairs = (int)std::max<size_t>(1, std::min<size_t>(npairs, budget / (w2_pair + w1_pair)));
     ctx->W1 = dev_alloc(ctx, w1_pair * ctx->cap_pairs);
     ctx->W2 = dev_alloc(ctx, w2_pair * ctx->cap_pairs);

@@ -167,6 +167,61 @@ def cpu_baseline(s, c0, seconds_hint):
             "s_per_step": dt, "warmup_s": t1 - t0}
 
 
+def run_md(cfg, prec, nsteps, dt_fs=0.5, temperature=300.0, gcut=1e-6):
+    """Born-Oppenheimer NVE MD (velocity Verlet) driven by the device SCF and
+    the adjoint forces; E_cons drift in Ha/atom/ps."""
+    from amdkiit_amd import drivers as D
+    from amdkiit_amd import systems
+    from amdkiit_amd.context import KSContext
+
+    s = systems.CONFIGS[cfg]()
+    rng = np.random.default_rng(7)
+    c0 = (rng.random((s.nbands, s.pw.ngpw)) - 0.5 + 1j * (rng.random((s.nbands, s.pw.ngpw)) - 0.5)) / (1.0 + s.pw.g2_pw)
+    c0[:, 0] = c0[:, 0].real
+    ctx = KSContext(s.pw, s.nbands, prec)
+    ctx.set_species(s.tables)
+    ctx.set_positions(s.sp, s.tau)
+    c = ctx.ortho(c0)
+    prov = D.GPUForceProvider(ctx, s, c, gcut=gcut)
+    m = np.array([D.MASS_AMU[s.species[i].symbol] for i in s.sp]) * D.AMU_AU
+    v = D.maxwell_boltzmann(m, temperature, seed=2024)
+    t0 = time.perf_counter()
+    h, drift, _, _ = D.bomd(prov, s.tau.copy(), m, v, dt_fs=dt_fs, nsteps=nsteps,
+                            log=lambda i, e, ek, ec: log(f"[bench]   md {i} E_cons {ec:.8f}") if i % 25 == 0 else None)
+    wall = time.perf_counter() - t0
+    ctx.close()
+    return {"config": cfg, "precision": prec, "steps": nsteps, "dt_fs": dt_fs, "T0_K": temperature,
+            "scf_gcut": gcut, "drift_Ha_per_atom_ps": drift,
+            "max_dev_Econs_Ha_per_atom": float(np.abs(h[:, 2] - h[0, 2]).max() / s.nat),
+            "scf_steps_first": prov.nscf[0], "scf_steps_per_md_step": float(np.mean(prov.nscf[1:])),
+            "wall_s": wall, "epot": h[:, 0].tolist()}
+
+
+def run_lbfgs(cfg, prec, nsteps, gcut=1e-6):
+    """SCF to GEMAX < gcut, then L-BFGS on the GPU adjoint forces."""
+    from amdkiit_amd import drivers as D
+    from amdkiit_amd import systems
+    from amdkiit_amd.context import KSContext
+
+    s = systems.CONFIGS[cfg]()
+    rng = np.random.default_rng(9)
+    c0 = (rng.random((s.nbands, s.pw.ngpw)) - 0.5 + 1j * (rng.random((s.nbands, s.pw.ngpw)) - 0.5)) / (1.0 + s.pw.g2_pw)
+    c0[:, 0] = c0[:, 0].real
+    ctx = KSContext(s.pw, s.nbands, prec)
+    ctx.set_species(s.tables)
+    ctx.set_positions(s.sp, s.tau)
+    c = ctx.ortho(c0)
+    prov = D.GPUForceProvider(ctx, s, c, gcut=gcut)
+    t0 = time.perf_counter()
+    _, hist = D.lbfgs(prov, s.tau.copy(), gtol=1e-4, maxiter=nsteps,
+                      log=lambda i, e, fm: log(f"[bench]   lbfgs {i} E {e:.8f} max|g| {fm:.2e}"))
+    wall = time.perf_counter() - t0
+    ctx.close()
+    return {"config": cfg, "precision": prec, "lbfgs_steps": len(hist) - 1, "scf_gcut": gcut,
+            "energy_force_history": hist, "scf_steps_total": int(np.sum(prov.nscf)),
+            "scf_steps_first": prov.nscf[0], "evaluations": len(prov.nscf), "wall_s": wall}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +233,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="skip the 64 H2O secondary measurement")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--md-steps", type=int, default=0, help="also run BO-MD (both precisions) on --md-config")
+    ap.add_argument("--md-config", default="h2o_64_pbe")
+    ap.add_argument("--lbfgs-steps", type=int, default=0, help="also run L-BFGS (mixed) on --lbfgs-config")
+    ap.add_argument("--lbfgs-config", default="h2o_128_pbe")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 512 MiB device-to-device copy (PMC byte-counter calibration)")
     a = ap.parse_args()
@@ -270,6 +329,23 @@ def main():
         log(f"[bench] 64 H2O: mixed {secondary['mixed_ms_per_step']:.2f} ms/step, fp64 speedup "
             f"{secondary['mixed_speedup']:.2f}x, H|psi> {secondary['hpsi_GBps']:.0f} GB/s")
 
+    md = None
+    if a.md_steps > 0:
+        md = {}
+        for prec in ("mixed", "fp64"):
+            log(f"[bench] BO-MD {a.md_config} {prec} {a.md_steps} steps ...")
+            md[prec] = run_md(a.md_config, prec, a.md_steps)
+            log(f"[bench]   drift {md[prec]['drift_Ha_per_atom_ps']:.2e} Ha/atom/ps, "
+                f"{md[prec]['wall_s'] / a.md_steps * 1e3:.1f} ms/MD step")
+        ep = np.array(md["mixed"]["epot"]) - np.array(md["fp64"]["epot"])
+        md["max_abs_dEpot_mixed_vs_fp64_Ha_per_atom"] = float(np.abs(ep).max() / len(systems.CONFIGS[a.md_config]().tau))
+    lb = None
+    if a.lbfgs_steps > 0:
+        log(f"[bench] L-BFGS {a.lbfgs_config} {a.lbfgs_steps} steps ...")
+        lb = run_lbfgs(a.lbfgs_config, "mixed", a.lbfgs_steps)
+        log(f"[bench]   E {lb['energy_force_history'][0][0]:.6f} -> {lb['energy_force_history'][-1][0]:.6f}, "
+            f"{lb['scf_steps_total']} SCF steps in {lb['wall_s']:.1f} s")
+
     cpu = None
     if not a.no_cpu:
         log("[bench] cpu baseline (FP64 port) ...")
@@ -305,6 +381,10 @@ def main():
         "secondary_64h2o": secondary,
         "cpu_baseline": cpu,
     }
+    if md:
+        out["bomd"] = md
+    if lb:
+        out["lbfgs"] = lb
     print(json.dumps(out))
 
 

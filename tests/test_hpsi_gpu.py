@@ -53,3 +53,25 @@ def test_density_matches_oracle(name, mk, nb, prec, tol):
     ref = O.eval_density(c, occ, pw.mill_pw, pw.n, pw.cell.omega)
     assert rel(got, ref) < tol
     ctx.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_chunked_pairs_match_single_pass(prec, monkeypatch):
+    """A small work-array budget splits the band pairs into several chunks
+    (W1/W2 reused per chunk); H|psi> and rho must not change."""
+    pw = PWSetup(Cell.cubic(18.63), 70.0)
+    nb = 11
+    c = random_bands(pw, nb, seed=5)
+    occ = np.full(nb, 2.0)
+    v = smooth_potential(pw)
+    out = {}
+    for budget in (None, "20"):
+        if budget:
+            monkeypatch.setenv("AMDK_PAIR_BUDGET_MB", budget)
+        ctx = KSContext(pw, nb, prec)
+        ctx.set_vks(v)
+        out[budget] = (ctx.vpsi(c, occ), ctx.eval_density(c, occ))
+        ctx.close()
+        monkeypatch.delenv("AMDK_PAIR_BUDGET_MB", raising=False)
+    assert np.array_equal(out[None][0], out["20"][0])
+    assert rel(out["20"][1], out[None][1]) < 1e-13

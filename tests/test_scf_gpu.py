@@ -202,3 +202,34 @@ def test_large_context_tables_and_first_step(prec):
     assert abs(e["enl"] - orc.e_nl(orc.fnl(c), s.occ)) < 1e-6 * abs(e["enl"])
     assert abs(e["nelec"] - s.occ.sum()) < 1e-10 * s.occ.sum()
     ctx.close()
+
+
+def test_mixed_matches_fp64_at_64_waters_converged():
+    """64 H2O PBE 80 Ry (144^3, 256 bands): SCF converged in both precisions
+    from the same start; mixed within 1e-6 Ha/atom of the FP64 path in E_tot
+    and within 1e-4 Ha/bohr in the forces, int rho = N_e to 1e-10."""
+    s = systems.CONFIGS["h2o_64_pbe"]()
+    c0 = random_bands(s.pw, s.nbands, seed=21)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        ctx = KSContext(s.pw, s.nbands, prec)
+        ctx.set_species(s.tables)
+        ctx.set_positions(s.sp, s.tau)
+        c = ctx.ortho(c0)
+        ctx.wfn_load(c, s.occ)
+        prev, f = np.inf, None
+        for it in range(1, 1500):
+            e, fi = ctx.scf_step_tfor(it, s.xc, prev < 3e-5)
+            f = fi if fi is not None else f
+            prev = e["gemax"]
+            if prev < 1e-5 and f is not None:
+                break
+        print(f"  {prec}: steps={it} etot={e['etot']:.8f} gemax={e['gemax']:.2e} nelec={e['nelec']:.12f}", flush=True)
+        res[prec] = (e, f)
+        assert abs(e["nelec"] - s.occ.sum()) < 1e-10 * s.occ.sum()
+        ctx.close()
+    de = abs(res["mixed"][0]["etot"] - res["fp64"][0]["etot"]) / s.nat
+    df = np.abs(res["mixed"][1] - res["fp64"][1]).max()
+    print(f"  |dE|/atom = {de:.2e}  max|dF| = {df:.2e}", flush=True)
+    assert de < 1e-6
+    assert df < 1e-4
