@@ -197,7 +197,7 @@ def run_md(cfg, prec, nsteps, dt_fs=0.5, temperature=300.0, gcut=1e-6):
             "wall_s": wall, "epot": h[:, 0].tolist()}
 
 
-def run_lbfgs(cfg, prec, nsteps, gcut=1e-6):
+def run_lbfgs(cfg, prec, nsteps, gcut=3e-5):
     """SCF to GEMAX < gcut, then L-BFGS on the GPU adjoint forces."""
     from amdkiit_amd import drivers as D
     from amdkiit_amd import systems
@@ -211,7 +211,7 @@ def run_lbfgs(cfg, prec, nsteps, gcut=1e-6):
     ctx.set_species(s.tables)
     ctx.set_positions(s.sp, s.tau)
     c = ctx.ortho(c0)
-    prov = D.GPUForceProvider(ctx, s, c, gcut=gcut)
+    prov = D.GPUForceProvider(ctx, s, c, gcut=gcut, maxit=6000)
     t0 = time.perf_counter()
     _, hist = D.lbfgs(prov, s.tau.copy(), gtol=1e-4, maxiter=nsteps,
                       log=lambda i, e, fm: log(f"[bench]   lbfgs {i} E {e:.8f} max|g| {fm:.2e}"))
