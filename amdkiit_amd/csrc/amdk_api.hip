@@ -293,8 +293,13 @@ static bool reg_ok(const amdk_ctx* c, int n) {
   }
 }
 
+// Launch shapes of the register passes: TX x columns per block (y and z
+// passes; 16 measured faster than 32 for FP32 100-point columns: more,
+// smaller workgroups per CU), HR half rows per block (x passes), PG band
+// pairs sharing one V(r) tile in the z pass of H|psi> (2, 4 and 8 measured
+// equal within 1 %).
 template <typename T, int TM> struct RegTiles {
-  static constexpr int TX = (sizeof(T) == 4 && TM <= 10) ? 32 : 16;
+  static constexpr int TX = 16;
   static constexpr int HR = 240 / (2 * TM);
   static constexpr int PG = sizeof(T) == 4 ? 4 : 2;
 };

@@ -262,9 +262,9 @@ __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X1,
   }
 }
 
-// Cholesky factor S = U^T U by one block of 32 x 32 threads with the matrix
-// in registers: thread (ti, tj) owns S(ti + 32 a, tj + 32 b), a, b < NPT
-// (n <= 32 NPT), upper triangle only.  Step k updates the trailing upper
+// Cholesky factor S = U^T U by one block of T x T threads with the matrix
+// in registers: thread (ti, tj) owns S(ti + T a, tj + T b), a, b < NPT
+// (n <= T NPT), upper triangle only.  Step k updates the trailing upper
 // triangle with deferred row scaling (a_ij -= a_ki a_kj / a_kk) from row k,
 // which its owners publish in a double-buffered LDS row at the end of step
 // k - 1 (one barrier per step); row blocks a that lie wholly above k are
@@ -272,10 +272,10 @@ __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X1,
 // (strict lower triangle zero) and Ut (n x n, contiguous) holds U row-major
 // (= U^T column-major) for k_trinv_cols.  A non-positive pivot k sets
 // info = info_base + k + 1 (info is never cleared here).
-template <int NPT>
-__global__ void __launch_bounds__(1024) k_chol_reg(double* __restrict__ A, int lda, double* __restrict__ Ut, int n,
-                                                   int* __restrict__ info, int info_base) {
-  constexpr int T = 32, NM = T * NPT;
+template <int T, int NPT>
+__global__ void __launch_bounds__(T * T) k_chol_reg(double* __restrict__ A, int lda, double* __restrict__ Ut, int n,
+                                                    int* __restrict__ info, int info_base) {
+  constexpr int NM = T * NPT;
   __shared__ double rowk[2][NM];
   __shared__ double diag[NM];
   const int ti = threadIdx.x % T, tj = threadIdx.x / T;
