@@ -488,13 +488,19 @@ static void launch_z_rho(amdk_ctx* c, const Sphere& S, int np, double* rho, int 
 }
 
 template <typename T>
-static void run_vpsi(amdk_ctx* c, const void* C0, void* C2, int nb) {
+// planes_ready: W2 already holds the y-transformed planes of all pairs of
+// these bands (left by run_density of the same C0 in one chunk), so the x and
+// y inverse passes are skipped.
+static void run_vpsi(amdk_ctx* c, const void* C0, void* C2, int nb, bool planes_ready = false) {
   const Sphere& S = c->spw;
   const int npairs = (nb + 1) / 2;
+  if (planes_ready && npairs > c->cap_pairs) throw Error(AMDK_ERR_STATE, "planes of all pairs do not fit the work arrays");
   for (int p0 = 0; p0 < npairs; p0 += c->cap_pairs) {
     const int np = std::min(c->cap_pairs, npairs - p0);
-    launch_rows_inv<T>(c, S, C0, S.ldc, nb, p0, np);
-    launch_cols<T>(c, S, np, true);
+    if (!planes_ready) {
+      launch_rows_inv<T>(c, S, C0, S.ldc, nb, p0, np);
+      launch_cols<T>(c, S, np, true);
+    }
     launch_z_vpsi<T>(c, S, np);
     launch_cols<T>(c, S, np, false);
     launch_rows_fwd_hpsi<T>(c, S, C0, C2, nb, p0, np);
