@@ -449,6 +449,19 @@ __global__ void __launch_bounds__(2 * HR * (A > B ? A : B))
   }
 }
 
+// Up to two field pairs per launch of the dense transforms (blockIdx.z / .y
+// selects the pair; work arrays W1/W2 hold one pair after the other).
+struct DenseIn {
+  const double* a[2];
+  const double* b[2];
+};
+template <typename T>
+struct DenseOut {
+  double* a[2];
+  double* b[2];
+  T* v[2];
+};
+
 
 // ---------------------------------------------------------------------------
 // Dense (density-sphere) passes in FP64: two real fields travel as a + i b.
@@ -456,15 +469,17 @@ __global__ void __launch_bounds__(2 * HR * (A > B ? A : B))
 // z-FFT exp(-i) of full columns of a(r) + i b(r); keep planes |m3| <= R.
 template <int A, int B, int TX>
 __global__ void __launch_bounds__(TX * (A > B ? A : B))
-    k_z_fwd_dense_r(PassGeom g, const double* __restrict__ a, const double* __restrict__ b,
-                    double2* __restrict__ W2, const double2* __restrict__ twg) {
+    k_z_fwd_dense_r(PassGeom g, DenseIn in, double2* __restrict__ W2, const double2* __restrict__ twg) {
   constexpr int N = A * B, TM = MaxAB<A, B>::v;
   extern __shared__ unsigned char smem_raw[];
   double2* tw = reinterpret_cast<double2*>(smem_raw);
   double2* lds = tw + N;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int n1 = g.n1, n2 = g.n2;
-  const int x = blockIdx.x * TX + c, y = blockIdx.y;
+  const int x = blockIdx.x * TX + c, y = blockIdx.y, f = blockIdx.z;  // field pair f
+  const double* __restrict__ a = in.a[f];
+  const double* __restrict__ b = in.b[f];
+  W2 += (size_t)f * g.nplane * g.n2 * g.p1;
   const bool live = x < n1;
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twg[i];
   __syncthreads();
@@ -492,15 +507,19 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
 // ov = scale * Re in type T (each may be null).
 template <typename T, int A, int B, int TX>
 __global__ void __launch_bounds__(TX * (A > B ? A : B))
-    k_z_inv_dense_r(PassGeom g, const double2* __restrict__ W2, double* __restrict__ oa, double* __restrict__ ob,
-                    T* __restrict__ ov, double scale, const double2* __restrict__ twg) {
+    k_z_inv_dense_r(PassGeom g, const double2* __restrict__ W2, DenseOut<T> out, double scale,
+                    const double2* __restrict__ twg) {
   constexpr int N = A * B, TM = MaxAB<A, B>::v;
   extern __shared__ unsigned char smem_raw[];
   double2* tw = reinterpret_cast<double2*>(smem_raw);
   double2* lds = tw + N;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int n1 = g.n1, n2 = g.n2;
-  const int x = blockIdx.x * TX + c, y = blockIdx.y;
+  const int x = blockIdx.x * TX + c, y = blockIdx.y, f = blockIdx.z;  // field pair f
+  double* __restrict__ oa = out.a[f];
+  double* __restrict__ ob = out.b[f];
+  T* __restrict__ ov = out.v[f];
+  W2 += (size_t)f * g.nplane * g.n2 * g.p1;
   const bool live = x < n1;
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twg[i];
   __syncthreads();
@@ -539,6 +558,8 @@ __global__ void __launch_bounds__(2 * HR * (A > B ? A : B))
   const int t = threadIdx.x % TM, fr = threadIdx.x / TM;
   const int hr = fr >> 1, mirror = fr & 1;
   const int row = blockIdx.x * HR + hr;
+  W1 += (size_t)blockIdx.y * g.nslot * g.p1;  // field pair blockIdx.y: its rows and its two output fields
+  out += (size_t)blockIdx.y * 2 * ldc;
   const bool live = row < g.nrow;
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twg[i];
   RowInfo ri{};
