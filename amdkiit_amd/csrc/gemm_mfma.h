@@ -162,14 +162,15 @@ constexpr int UPM_MC = 32, UPM_NB = 128, UPM_K = 64;
 // A second operand pair (X2: M2 rows, Q2) may follow the first: the sum runs
 // over the stacked rows (part 1 padded with zero rows to a whole 32-row
 // chunk), with Q1 scaled by a1 and Q2 by a2, so that two updates of Y share
-// one pass over it.
+// one pass over it.  zero_g0_im stores 0 into real row 1 of every column
+// (Im c(G = 0) of a Gamma-point band, which must vanish).
 template <typename TC>
 __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X1, int ldx, int M1,
                                                   const double* __restrict__ Q1, int ldq1, double a1,
                                                   const double* __restrict__ X2, int M2,
                                                   const double* __restrict__ Q2, int ldq2, double a2,
                                                   double* __restrict__ Y, int ldy, int N, int K,
-                                                  double beta, int per) {
+                                                  double beta, int per, int zero_g0_im) {
   typedef typename Acc4<TC>::type AccT;
   constexpr bool F64 = sizeof(TC) == 8;
   constexpr int XL = (UPM_MC * UPM_K / 2) / 256;   // double2 of X per thread
@@ -256,7 +257,7 @@ __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X1,
         const int n = n0 + 16 * t + acc_row<TC>(lk, v);
         if (n < N) {
           double* y = Y + (size_t)n * ldy + k;
-          *y = (beta != 0.0 ? beta * *y : 0.0) + acc[t][v];
+          *y = (zero_g0_im && k == 1) ? 0.0 : (beta != 0.0 ? beta * *y : 0.0) + acc[t][v];
         }
       }
   }
@@ -271,7 +272,7 @@ __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X1,
 // skipped.  A has leading dimension lda; on return its n x n block holds U
 // (strict lower triangle zero) and Ut (n x n, contiguous) holds U row-major
 // (= U^T column-major) for k_trinv_cols.  A non-positive pivot k sets
-// info = info_base + k + 1 (info is never cleared here).
+// info = info_base + k + 1; the block with info_base 0 first clears info.
 template <int T, int NPT>
 __global__ void __launch_bounds__(T * T) k_chol_reg(double* __restrict__ A, int lda, double* __restrict__ Ut, int n,
                                                     int* __restrict__ info, int info_base) {
@@ -280,6 +281,7 @@ __global__ void __launch_bounds__(T * T) k_chol_reg(double* __restrict__ A, int 
   __shared__ double diag[NM];
   const int ti = threadIdx.x % T, tj = threadIdx.x / T;
   double v[NPT][NPT];
+  if (info_base == 0 && threadIdx.x == 0) *info = 0;
 #pragma unroll
   for (int a = 0; a < NPT; ++a)
 #pragma unroll
@@ -338,14 +340,6 @@ __global__ void __launch_bounds__(T * T) k_chol_reg(double* __restrict__ A, int 
     }
 }
 
-// Ut = U row-major from a factor left in the upper triangle of A by potrf.
-__global__ void k_upper_to_rows(const double* __restrict__ A, double* __restrict__ Ut, int n) {
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < (size_t)n * n; e += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(e / n), j = (int)(e % n);  // Ut[i][j] = U(i, j)
-    Ut[e] = i <= j ? A[i + (size_t)j * n] : 0.0;
-  }
-}
-
 // value of v in lane l (l uniform across the wave)
 __device__ __forceinline__ double bcast_lane(double v, int l) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
@@ -361,7 +355,8 @@ __device__ __forceinline__ double bcast_lane(double v, int l) {
 // X may be the storage of U (column-major): column j is written only by
 // block j and U is read from Ut.  n <= NMAX.
 template <int NMAX, int C>
-__global__ void __launch_bounds__(256) k_trinv_cols(const double* __restrict__ Ut, int n, double* __restrict__ X, int ldx) {
+__global__ void __launch_bounds__(256) k_trinv_cols(const double* __restrict__ Ut, int n, double* __restrict__ X, int ldx,
+                                                    int nz) {
   static_assert(C <= 64 && 256 % C == 0, "chunk rows");
   __shared__ double x[NMAX];
   __shared__ double uc[C][NMAX + 1];
@@ -409,7 +404,7 @@ __global__ void __launch_bounds__(256) k_trinv_cols(const double* __restrict__ U
     __syncthreads();
     i1 = i0;
   }
-  for (int k = tid; k < n; k += 256) X[k + (size_t)j * ldx] = x[k];
+  for (int k = tid; k < nz; k += 256) X[k + (size_t)j * ldx] = k < n ? x[k] : 0.0;
 }
 
 // C = alpha op(A) op(B) + beta C for the small band-block products of the
@@ -441,16 +436,6 @@ __global__ void __launch_bounds__(256) k_gemm_small(int ta, int tb, int M, int N
     double* c = C + i + (size_t)j * ldc;
     *c = alpha * acc + (beta != 0.0 ? beta * *c : 0.0);
   }
-}
-
-// X(0:m, 0:n) = 0 (column-major, leading dimension ldx)
-__global__ void k_zero_block(double* __restrict__ X, int ldx, int m, int n) {
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < (size_t)m * n; e += (size_t)gridDim.x * blockDim.x)
-    X[(e % m) + (e / m) * (size_t)ldx] = 0.0;
-}
-
-__global__ void k_set_int(int* __restrict__ p, int v) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
 }  // namespace amdk

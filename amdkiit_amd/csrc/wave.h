@@ -34,10 +34,13 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
 
 constexpr int RED_BLOCKS = 512;
 
-// Final pass: out[q] = sum (or max) over RED_BLOCKS partials of quantity q.
-__global__ void k_red_final(const double* __restrict__ part, int nq, double* __restrict__ out, int is_max) {
+// Final pass over the slots set in `slots` (bit q): out[q] = sum, or max when
+// bit q of `maxes` is set, of the RED_BLOCKS partials of slot q.
+__global__ void k_red_slots(const double* __restrict__ part, unsigned slots, unsigned maxes, double* __restrict__ out) {
   __shared__ double sh[4];
-  for (int q = 0; q < nq; ++q) {
+  for (int q = 0; q < 16; ++q) {
+    if (!((slots >> q) & 1u)) continue;
+    const bool is_max = (maxes >> q) & 1u;
     double v = is_max ? -1.0 : 0.0;
     for (int i = threadIdx.x; i < RED_BLOCKS; i += blockDim.x) {
       const double x = part[(size_t)q * RED_BLOCKS + i];
@@ -105,12 +108,6 @@ __global__ void __launch_bounds__(256) k_grad_norms(const V* __restrict__ X, int
     part6[blockIdx.x] = rm;
     part7[blockIdx.x] = rp;
   }
-}
-
-template <typename V>
-__global__ void k_zclean(V* __restrict__ C, int ldc, int nb) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < nb) C[(size_t)b * ldc].y = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -212,10 +209,16 @@ __global__ void k_dbeta(const V* __restrict__ beta, const int* __restrict__ mill
 // ---------------------------------------------------------------------------
 // PCG (cg.f90:63-145): P = f^-1 K(G)^-1 C2 (its norm from k_grad_norms, P
 // itself formed inside k_pcg_update), d = P + gamma d_prev, C0 += step d.
-// scalars: st[0] = ||P||^2 of this step (input), st[1] = previous ||P||^2,
-// st[2] = gamma out.  istep is 1-based within one optimisation.
-__global__ void k_pcg_scalar(double* st, int istep) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
+// k_pcg_scalar: st[0] = ||P||^2 of this step (the sum of its RED_BLOCKS
+// partials), st[2] = gamma = forget * st[0] / st[1] (0 at istep 1), then
+// st[1] = st[0].  istep is 1-based within one optimisation.
+__global__ void k_pcg_scalar(const double* __restrict__ part, double* st, int istep) {
+  __shared__ double sh[4];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < RED_BLOCKS; i += blockDim.x) v += part[i];
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) {
+    st[0] = v;
     double gamma = 0.0;
     if (istep > 1) {
       const double forget = istep <= 10 ? 0.60 : (istep <= 20 ? 0.91 : 0.95);
@@ -252,14 +255,6 @@ __global__ void k_pcg_update(V* __restrict__ C0, V* __restrict__ H, const V* __r
     c.x = (double)c.x + step * (double)d.x;
     c.y = (double)c.y + step * (double)d.y;
     C0[i] = c;
-  }
-}
-
-// occupation masking of the Lagrange matrix.
-__global__ void k_mask_occ(double* A, int n, const double* __restrict__ occ) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n * n; i += gridDim.x * blockDim.x) {
-    const int r = i % n, c = i / n;
-    if (occ[r] == 0.0 || occ[c] == 0.0) A[i] = 0.0;
   }
 }
 
