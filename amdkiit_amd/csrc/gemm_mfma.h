@@ -263,83 +263,6 @@ __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X1,
   }
 }
 
-// Cholesky factor S = U^T U by one block of T x T threads with the matrix
-// in registers: thread (ti, tj) owns S(ti + T a, tj + T b), a, b < NPT
-// (n <= T NPT), upper triangle only.  Step k updates the trailing upper
-// triangle with deferred row scaling (a_ij -= a_ki a_kj / a_kk) from row k,
-// which its owners publish in a double-buffered LDS row at the end of step
-// k - 1 (one barrier per step); row blocks a that lie wholly above k are
-// skipped.  A has leading dimension lda; on return its n x n block holds U
-// (strict lower triangle zero) and Ut (n x n, contiguous) holds U row-major
-// (= U^T column-major) for k_trinv_cols.  A non-positive pivot k sets
-// info = info_base + k + 1; the block with info_base 0 first clears info.
-template <int T, int NPT>
-__global__ void __launch_bounds__(T * T) k_chol_reg(double* __restrict__ A, int lda, double* __restrict__ Ut, int n,
-                                                    int* __restrict__ info, int info_base) {
-  constexpr int NM = T * NPT;
-  __shared__ double rowk[2][NM];
-  __shared__ double diag[NM];
-  const int ti = threadIdx.x % T, tj = threadIdx.x / T;
-  double v[NPT][NPT];
-  if (info_base == 0 && threadIdx.x == 0) *info = 0;
-#pragma unroll
-  for (int a = 0; a < NPT; ++a)
-#pragma unroll
-    for (int b = 0; b < NPT; ++b) {
-      const int i = ti + T * a, j = tj + T * b;
-      v[a][b] = (i <= j && j < n) ? A[i + (size_t)j * lda] : 0.0;
-    }
-  auto publish = [&](int k, double* dst) {  // owners of row k write it
-    if (ti == k % T) {
-#pragma unroll
-      for (int a = 0; a < NPT; ++a)
-        if (a == k / T) {
-#pragma unroll
-          for (int b = 0; b < NPT; ++b) dst[tj + T * b] = v[a][b];
-        }
-    }
-  };
-  publish(0, rowk[0]);
-  __syncthreads();
-  for (int k = 0; k < n; ++k) {
-    const double* rk = rowk[k & 1];
-    const double akk = rk[k];
-    if (!(akk > 0.0)) {  // the same value in every thread
-      if (threadIdx.x == 0) *info = info_base + k + 1;
-      return;
-    }
-    const double inv = 1.0 / akk;
-#pragma unroll
-    for (int a = 0; a < NPT; ++a) {
-      if (T * a + T - 1 <= k) continue;  // rows of this block all done (uniform)
-      const int i = ti + T * a;
-      const double fi = rk[i] * inv;
-#pragma unroll
-      for (int b = 0; b < NPT; ++b) {
-        const int j = tj + T * b;
-        if (i > k && i <= j && j < n) v[a][b] -= fi * rk[j];
-      }
-    }
-    if (k + 1 < n) publish(k + 1, rowk[(k + 1) & 1]);
-    __syncthreads();
-  }
-#pragma unroll
-  for (int a = 0; a < NPT; ++a)
-    if (ti == tj && ti + T * a < n) diag[ti + T * a] = 1.0 / sqrt(v[a][a]);
-  __syncthreads();
-#pragma unroll
-  for (int a = 0; a < NPT; ++a)
-#pragma unroll
-    for (int b = 0; b < NPT; ++b) {
-      const int i = ti + T * a, j = tj + T * b;
-      if (i < n && j < n) {
-        const double u = i <= j ? v[a][b] * diag[i] : 0.0;
-        A[i + (size_t)j * lda] = u;
-        Ut[(size_t)i * n + j] = u;
-      }
-    }
-}
-
 // value of v in lane l (l uniform across the wave)
 __device__ __forceinline__ double bcast_lane(double v, int l) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
@@ -347,65 +270,208 @@ __device__ __forceinline__ double bcast_lane(double v, int l) {
   return __hiloint2double(hi, lo);
 }
 
-// X = U^-1, one block of 256 threads per column j: back substitution
-// U x = e_j, x_i = -(sum_{k=i+1..j} U_ik x_k) / U_ii, in chunks of C rows
-// (descending).  Per chunk the rows of U (from Ut, row-major) are staged in
-// LDS, the part of each row's sum over already known x is formed by all
-// threads, and one wave finishes the C x C triangle, one row per lane.
-// X may be the storage of U (column-major): column j is written only by
-// block j and U is read from Ut.  n <= NMAX.
-template <int NMAX, int C>
-__global__ void __launch_bounds__(256) k_trinv_cols(const double* __restrict__ Ut, int n, double* __restrict__ X, int ldx,
-                                                    int nz) {
-  static_assert(C <= 64 && 256 % C == 0, "chunk rows");
-  __shared__ double x[NMAX];
-  __shared__ double uc[C][NMAX + 1];
-  __shared__ double part[C];
-  __shared__ double red[256];
-  const int j = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int k = tid; k < NMAX; k += 256) x[k] = 0.0;
-  __syncthreads();
-  if (tid == 0) x[j] = 1.0 / Ut[(size_t)j * n + j];
-  __syncthreads();
-  for (int i1 = j; i1 > 0;) {
-    const int i0 = max(0, i1 - C), rows = i1 - i0, w = j - i0 + 1;  // columns i0..j
-    for (int e = tid; e < rows * w; e += 256) {
-      const int r = e / w, kk = e % w;
-      uc[r][kk] = Ut[(size_t)(i0 + r) * n + i0 + kk];
-    }
-    __syncthreads();
-    // part[r] = sum_{k = i1..j} U(i0 + r, k) x_k by 256 / C threads per row
-    {
-      constexpr int TPR = 256 / C;
-      const int r = tid / TPR, q = tid % TPR;
-      double s = 0.0;
-      if (r < rows)
-        for (int k = i1 + q; k <= j; k += TPR) s += uc[r][k - i0] * x[k];
-      red[tid] = s;
-      __syncthreads();
-      if (q == 0 && r < rows) {
-        double t = 0.0;
-        for (int u = 0; u < TPR; ++u) t += red[tid + u];
-        part[r] = t;
-      }
-      __syncthreads();
-    }
-    if (wave == 0) {
-      // lane r keeps the running sum of row i0 + r; each x_l, once known in
-      // lane l, is broadcast with readlane and folded into the rows above
-      double acc = lane < rows ? part[lane] : 0.0, xl = 0.0;
-      for (int l = rows - 1; l >= 0; --l) {
-        if (lane == l) xl = -acc / uc[l][l];
-        const double xb = bcast_lane(xl, l);
-        if (lane < l) acc += uc[lane][l] * xb;
-      }
-      if (lane < rows) x[i0 + lane] = xl;
-    }
-    __syncthreads();
-    i1 = i0;
+// S = U^T U for the upper triangle of the n x n block at A (lda; read
+// only) and X = U^-1 into rows [0, nz) of columns [0, n) of X (ldx; zero
+// below the diagonal), by one block of NW waves, n <= 16 NT.
+// Elimination step k subtracts (s_ki / s_kk) times row k from each row
+// i > k, both in the upper triangle (s, diagonal included) and in the
+// strictly lower triangle, which starts as the zeros of the identity (with
+// its unit diagonal implicit) and ends as M = L^-1 of S = L D L^T.  Then
+// d_c = s_cc and U^-1(r, c) = M(c, r) / sqrt(d_c) for r < c, 1 / sqrt(d_c)
+// for r = c.  The array lives in MFMA accumulators as 16 x 16 tiles: upper
+// tiles (J >= I) hold s, lower tiles (J <= I) hold M, the diagonal tiles
+// both (each half read from its own copy); the waves share the tiles
+// round-robin.
+// Steps run in panels of PB rows, two barriers per panel.  The owners
+// publish the panel's rows to LDS.  The waves that hold columns (thread
+// t < n: column t) eliminate inside the panel: lane l < PB repeats it on the
+// panel's own column k0 + l, where row k_q's lower part reads 1 at k_q and 0
+// beyond, with pivots and multipliers taken from those lanes by readlane
+// (the same values in every such wave); every other column takes the same
+// row operations.  The eliminated rows R go to LDS, with the lower-part view
+// RM (R before k_q, 1 at k_q, 0 after).  Then each tile below the panel
+// takes the panel's PB row operations as one rank-PB MFMA update,
+// T -= F R (upper) or T -= F RM (lower), F(i, q) = R(q, i) / d_q, and tiles
+// that it leaves unchanged are skipped.  A non-positive pivot k sets
+// info = info_base + k + 1 and the whole block returns; the block with
+// info_base 0 first clears info.
+template <int NT, int NW, int PB>
+__global__ void __launch_bounds__(64 * NW) k_chol_inv(const double* __restrict__ A, int lda, int n,
+                                                     double* __restrict__ X, int ldx, int nz,
+                                                     int* __restrict__ info, int info_base) {
+  constexpr int NM = 16 * NT, NS = NT * (NT + 1) / 2, SL = (2 * NS + NW - 1) / NW, NMP = NM + 2;
+  static_assert(PB % 4 == 0 && 16 % PB == 0, "panels of 4, 8 or 16 rows inside one tile row");
+  static_assert(64 * NW >= NM, "one thread per column");
+  __shared__ double raw[PB][NM];
+  __shared__ double R[PB][NMP];
+  __shared__ double RM[PB][NMP];
+  __shared__ double dinv_p[PB];
+  __shared__ double dinv[NM];
+  __shared__ int fail;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int li = lane & 15, lk = lane >> 4;
+  if (tid == 0) {
+    fail = 0;
+    if (info_base == 0) *info = 0;
   }
-  for (int k = tid; k < nz; k += 256) X[k + (size_t)j * ldx] = k < n ? x[k] : 0.0;
+  for (int e = tid; e < PB * NMP; e += 64 * NW) {  // columns >= n stay zero
+    (&R[0][0])[e] = 0.0;
+    (&RM[0][0])[e] = 0.0;
+  }
+  // tiles of this wave: t = wave + NW s; t < NS upper (row-major over I <= J),
+  // else lower (transposed); code = I | J << 8 | upper << 16 | live << 17
+  int code[SL];
+  dbl4 acc[SL];
+#pragma unroll
+  for (int sl = 0; sl < SL; ++sl) {
+    const int t = wave + NW * sl;
+    int u = t < NS ? t : t - NS, I = 0;
+    while (I < NT - 1 && u >= NT - I) {
+      u -= NT - I;
+      ++I;
+    }
+    const int J = I + u;
+    const bool upper = t < NS;
+    const int TI = upper ? I : J, TJ = upper ? J : I;
+    const bool lv = t < 2 * NS && 16 * TI < n && 16 * TJ < n;
+    code[sl] = TI | TJ << 8 | (upper ? 1 << 16 : 0) | (lv ? 1 << 17 : 0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = 16 * TI + 4 * v + lk, x = 16 * TJ + li;
+      acc[sl][v] = (lv && upper && i <= x && x < n) ? A[i + (size_t)x * lda] : 0.0;
+    }
+  }
+#define TILE_I(sl) (code[sl] & 255)
+#define TILE_J(sl) ((code[sl] >> 8) & 255)
+#define TILE_UP(sl) ((code[sl] >> 16) & 1)
+#define TILE_LIVE(sl) ((code[sl] >> 17) & 1)
+  // raw[p][x] = row k0 + p of the combined array (upper part from upper tiles)
+  auto publish = [&](int k0) {
+#pragma unroll
+    for (int sl = 0; sl < SL; ++sl) {
+      if (!TILE_LIVE(sl) || TILE_I(sl) != k0 / 16) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = 16 * TILE_I(sl) + 4 * v + lk, x = 16 * TILE_J(sl) + li, p = i - k0;
+        if (p >= 0 && p < PB && (TILE_UP(sl) ? x >= i : x < i)) raw[p][x] = acc[sl][v];
+      }
+    }
+  };
+  __syncthreads();
+  publish(0);
+  __syncthreads();
+  for (int k0 = 0; k0 < n; k0 += PB) {
+    const int pn = min(PB, n - k0);
+    if (tid - lane < n) {  // waves holding columns
+      const int x = tid, y = k0 + lane;
+      double col[PB], pc[PB];
+#pragma unroll
+      for (int c = 0; c < PB; ++c) {
+        col[c] = (x < n && c < pn) ? raw[c][x] : 0.0;
+        pc[c] = (lane < pn && c < pn) ? raw[c][min(y, NM - 1)] : 0.0;
+      }
+      int bad = PB;  // first non-positive pivot of the panel
+      double dinv_q[PB];
+      // rows q >= pn (past n) are zero and run with pivot 1, so the loops have
+      // no data-dependent exits and the independent updates interleave
+#pragma unroll
+      for (int q = 0; q < PB; ++q) {
+        const double dp = bcast_lane(pc[q], q);  // R_q(k_q), lane q
+        if (q < pn && !(dp > 0.0) && bad == PB) bad = q;
+        const double dq = q < pn ? dp : 1.0;
+        // 1 / d_q: hardware estimate and two Newton steps
+        double inv = __builtin_amdgcn_rcp(dq);
+        inv = fma(inv, fma(-dq, inv, 1.0), inv);
+        inv = fma(inv, fma(-dq, inv, 1.0), inv);
+        dinv_q[q] = inv;
+        const int kq = k0 + q;
+#pragma unroll
+        for (int c = q + 1; c < PB; ++c) {
+          const double f = bcast_lane(pc[q], c) * inv;  // R_q(k_c) / d_q, lane c
+          const int kc = k0 + c;
+          pc[c] -= f * ((y < kq || y >= kc) ? pc[q] : (y == kq ? 1.0 : 0.0));
+          col[c] -= f * col[q];  // exact for columns outside the panel
+        }
+      }
+      if (bad < PB) {  // the same in every column wave: the block returns after the barrier
+        if (tid == 0) {
+          *info = info_base + k0 + bad + 1;
+          fail = 1;
+        }
+      } else {
+        const bool in_panel = x >= k0 && x < k0 + PB;
+#pragma unroll
+        for (int c = 0; c < PB; ++c) {
+          if (c >= pn) continue;
+          if (x < n && !in_panel) {
+            R[c][x] = col[c];
+            RM[c][x] = x < k0 ? col[c] : 0.0;
+          }
+          if (tid < pn) {  // wave 0 writes the panel's columns
+            R[c][k0 + tid] = pc[c];
+            RM[c][k0 + tid] = tid < c ? pc[c] : (tid == c ? 1.0 : 0.0);
+          }
+          if (tid == 0) dinv_p[c] = dinv_q[c];
+        }
+      }
+    }
+    __syncthreads();
+    if (fail) return;  // the whole block
+#pragma unroll
+    for (int sl = 0; sl < SL; ++sl) {
+      // rows below the panel only; lower tiles right of the panel stay unchanged
+      if (!TILE_LIVE(sl) || 16 * TILE_I(sl) + 15 < k0 + pn || (!TILE_UP(sl) && 16 * TILE_J(sl) > k0 + PB - 1)) continue;
+      const int i = 16 * TILE_I(sl) + li, x = 16 * TILE_J(sl) + li;
+      const bool below = i >= k0 + pn && i < n;
+      const double(*E)[NMP] = TILE_UP(sl) ? R : RM;
+#pragma unroll
+      for (int q0 = 0; q0 < PB; q0 += 4) {
+        const int q = q0 + lk;
+        const double a = (below && q < pn) ? -R[q][i] * dinv_p[q] : 0.0;
+        acc[sl] = mfma16(a, E[q][x], acc[sl]);
+      }
+    }
+    // the panel's rows take their eliminated values
+#pragma unroll
+    for (int sl = 0; sl < SL; ++sl) {
+      if (!TILE_LIVE(sl) || TILE_I(sl) != k0 / 16) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = 16 * TILE_I(sl) + 4 * v + lk, x = 16 * TILE_J(sl) + li, p = i - k0;
+        if (p >= 0 && p < pn && x < n) acc[sl][v] = R[p][x];
+      }
+    }
+    if (k0 + PB < n) publish(k0 + PB);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int sl = 0; sl < SL; ++sl) {
+    if (!TILE_LIVE(sl) || !TILE_UP(sl) || TILE_I(sl) != TILE_J(sl)) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = 16 * TILE_I(sl) + 4 * v + lk, x = 16 * TILE_J(sl) + li;
+      if (x == i && i < n) dinv[i] = 1.0 / sqrt(acc[sl][v]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int sl = 0; sl < SL; ++sl) {
+    if (!TILE_LIVE(sl)) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int c = 16 * TILE_I(sl) + 4 * v + lk, r = 16 * TILE_J(sl) + li;  // X(r, c) from element (c, r)
+      if (c >= n || r >= n) continue;
+      if (TILE_UP(sl) && r >= c) X[r + (size_t)c * ldx] = r == c ? dinv[c] : 0.0;
+      if (!TILE_UP(sl) && r < c) X[r + (size_t)c * ldx] = acc[sl][v] * dinv[c];
+    }
+  }
+  const int zr = nz - n;
+  for (int e = tid; e < zr * n; e += 64 * NW) X[n + e % zr + (size_t)(e / zr) * ldx] = 0.0;
 }
+#undef TILE_I
+#undef TILE_J
+#undef TILE_UP
+#undef TILE_LIVE
 
 // C = alpha op(A) op(B) + beta C for the small band-block products of the
 // blocked Cholesky (FP64, 16 x 16 output tiles, K in LDS chunks of 16);

@@ -95,6 +95,37 @@ def test_ortho_rejects_dependent_bands():
     ctx.close()
 
 
+@pytest.mark.parametrize("nb", [1, 37, 64, 65, 128, 129, 200, 300])
+def test_ortho_band_counts(nb):
+    """Cholesky orthonormalisation at band counts around the one-workgroup
+    limit (128) and the blocked split: U^-1 against the oracle, C^T C = 1."""
+    s = SMALL["h2o2_pz"]()
+    ctx = KSContext(s.pw, nb, "fp64")
+    orc = KSOracle(s.pw, s.tables, s.sp, s.tau, s.xc, workers=8)
+    print(f"case=ortho nb={nb} npw={s.pw.ngpw}", flush=True)
+    raw = random_bands(s.pw, nb, seed=40 + nb)
+    o = ctx.ortho(raw)
+    assert rel(o, orc.ortho(raw)) < 1e-10
+    S = orc.ovlap(o, o)
+    assert np.abs(S - np.eye(nb)).max() < 1e-12
+    assert np.all(o[:, 0].imag == 0.0)
+    ctx.close()
+
+
+def test_ortho_reports_the_failing_band_beyond_the_first_block():
+    s = SMALL["h2o2_pz"]()
+    nb = 200
+    ctx = KSContext(s.pw, nb, "fp64")
+    c = random_bands(s.pw, nb, seed=9)
+    c[150] = 0.0  # a null band: pivot 151 of S (in the second block) is exactly zero
+    from amdkiit_amd.lib import AmdkError, ERR_NOT_PD
+    with pytest.raises(AmdkError) as ei:
+        ctx.ortho(c)
+    assert ei.value.code == ERR_NOT_PD
+    assert "leading minor 151)" in str(ei.value)
+    ctx.close()
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_vks_and_energies(sysname, prec):
     s, ctx, orc = build(SMALL[sysname], prec)
