@@ -272,16 +272,15 @@ __device__ __forceinline__ double bcast_lane(double v, int l) {
 
 // S = U^T U for the upper triangle of the n x n block at A (lda; read
 // only) and X = U^-1 into rows [0, nz) of columns [0, n) of X (ldx; zero
-// below the diagonal), by one block of NW waves, n <= 16 NT.
+// below the diagonal), by one block of NT waves, n <= 16 NT.
 // Elimination step k subtracts (s_ki / s_kk) times row k from each row
 // i > k, both in the upper triangle (s, diagonal included) and in the
 // strictly lower triangle, which starts as the zeros of the identity (with
 // its unit diagonal implicit) and ends as M = L^-1 of S = L D L^T.  Then
 // d_c = s_cc and U^-1(r, c) = M(c, r) / sqrt(d_c) for r < c, 1 / sqrt(d_c)
-// for r = c.  The array lives in MFMA accumulators as 16 x 16 tiles: upper
-// tiles (J >= I) hold s, lower tiles (J <= I) hold M, the diagonal tiles
-// both (each half read from its own copy); the waves share the tiles
-// round-robin.
+// for r = c.  The array lives in MFMA accumulators as 16 x 16 tiles; wave J
+// holds tile column J: tiles (I, J) above the diagonal hold s, below it M,
+// and the diagonal tile has one copy of each (each half read from its own).
 // Steps run in panels of PB rows, two barriers per panel.  The owners
 // publish the panel's rows to LDS.  The waves that hold columns (thread
 // t < n: column t) eliminate inside the panel: lane l < PB repeats it on the
@@ -295,65 +294,48 @@ __device__ __forceinline__ double bcast_lane(double v, int l) {
 // that it leaves unchanged are skipped.  A non-positive pivot k sets
 // info = info_base + k + 1 and the whole block returns; the block with
 // info_base 0 first clears info.
-template <int NT, int NW, int PB>
-__global__ void __launch_bounds__(64 * NW) k_chol_inv(const double* __restrict__ A, int lda, int n,
+template <int NT, int PB>
+__global__ void __launch_bounds__(64 * NT) k_chol_inv(const double* __restrict__ A, int lda, int n,
                                                      double* __restrict__ X, int ldx, int nz,
                                                      int* __restrict__ info, int info_base) {
-  constexpr int NM = 16 * NT, NS = NT * (NT + 1) / 2, SL = (2 * NS + NW - 1) / NW, NMP = NM + 2;
+  constexpr int NM = 16 * NT, NMP = NM + 2;
   static_assert(PB % 4 == 0 && 16 % PB == 0, "panels of 4, 8 or 16 rows inside one tile row");
-  static_assert(64 * NW >= NM, "one thread per column");
   __shared__ double raw[PB][NM];
   __shared__ double R[PB][NMP];
   __shared__ double RM[PB][NMP];
   __shared__ double dinv_p[PB];
   __shared__ double dinv[NM];
   __shared__ int fail;
-  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int li = lane & 15, lk = lane >> 4;
+  const int tid = threadIdx.x, lane = tid & 63, J = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int li = lane & 15, lk = lane >> 4, xj = 16 * J + li;  // this lane's column in its tiles
+  const bool jlive = 16 * J < n;
   if (tid == 0) {
     fail = 0;
     if (info_base == 0) *info = 0;
   }
-  for (int e = tid; e < PB * NMP; e += 64 * NW) {  // columns >= n stay zero
+  for (int e = tid; e < PB * NMP; e += 64 * NT) {  // columns >= n stay zero
     (&R[0][0])[e] = 0.0;
     (&RM[0][0])[e] = 0.0;
   }
-  // tiles of this wave: t = wave + NW s; t < NS upper (row-major over I <= J),
-  // else lower (transposed); code = I | J << 8 | upper << 16 | live << 17
-  int code[SL];
-  dbl4 acc[SL];
+  dbl4 acc[NT], accd;  // tile (I, J) for I = 0 .. NT-1; accd: lower copy of (J, J)
 #pragma unroll
-  for (int sl = 0; sl < SL; ++sl) {
-    const int t = wave + NW * sl;
-    int u = t < NS ? t : t - NS, I = 0;
-    while (I < NT - 1 && u >= NT - I) {
-      u -= NT - I;
-      ++I;
-    }
-    const int J = I + u;
-    const bool upper = t < NS;
-    const int TI = upper ? I : J, TJ = upper ? J : I;
-    const bool lv = t < 2 * NS && 16 * TI < n && 16 * TJ < n;
-    code[sl] = TI | TJ << 8 | (upper ? 1 << 16 : 0) | (lv ? 1 << 17 : 0);
+  for (int I = 0; I < NT; ++I)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int i = 16 * TI + 4 * v + lk, x = 16 * TJ + li;
-      acc[sl][v] = (lv && upper && i <= x && x < n) ? A[i + (size_t)x * lda] : 0.0;
+      const int i = 16 * I + 4 * v + lk;
+      acc[I][v] = (I <= J && i <= xj && xj < n) ? A[i + (size_t)xj * lda] : 0.0;
     }
-  }
-#define TILE_I(sl) (code[sl] & 255)
-#define TILE_J(sl) ((code[sl] >> 8) & 255)
-#define TILE_UP(sl) ((code[sl] >> 16) & 1)
-#define TILE_LIVE(sl) ((code[sl] >> 17) & 1)
-  // raw[p][x] = row k0 + p of the combined array (upper part from upper tiles)
+  accd = (dbl4){0.0, 0.0, 0.0, 0.0};
+  // raw[p][x] = row k0 + p of the combined array, from tile row k0 / 16
   auto publish = [&](int k0) {
+    if (!jlive) return;
 #pragma unroll
-    for (int sl = 0; sl < SL; ++sl) {
-      if (!TILE_LIVE(sl) || TILE_I(sl) != k0 / 16) continue;
+    for (int I = 0; I < NT; ++I) {
+      if (I != k0 / 16) continue;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int i = 16 * TILE_I(sl) + 4 * v + lk, x = 16 * TILE_J(sl) + li, p = i - k0;
-        if (p >= 0 && p < PB && (TILE_UP(sl) ? x >= i : x < i)) raw[p][x] = acc[sl][v];
+        const int i = 16 * I + 4 * v + lk, p = i - k0;
+        if (p >= 0 && p < PB) raw[p][xj] = (I == J && xj < i) ? accd[v] : acc[I][v];
       }
     }
   };
@@ -417,61 +399,73 @@ __global__ void __launch_bounds__(64 * NW) k_chol_inv(const double* __restrict__
     }
     __syncthreads();
     if (fail) return;  // the whole block
+    if (jlive) {
+      // operand rows of F for tile row I: i = 16 I + li, panel step q = q0 + lk
+      const bool lower_live = 16 * J <= k0 + PB - 1;  // lower tiles right of the panel stay unchanged
 #pragma unroll
-    for (int sl = 0; sl < SL; ++sl) {
-      // rows below the panel only; lower tiles right of the panel stay unchanged
-      if (!TILE_LIVE(sl) || 16 * TILE_I(sl) + 15 < k0 + pn || (!TILE_UP(sl) && 16 * TILE_J(sl) > k0 + PB - 1)) continue;
-      const int i = 16 * TILE_I(sl) + li, x = 16 * TILE_J(sl) + li;
-      const bool below = i >= k0 + pn && i < n;
-      const double(*E)[NMP] = TILE_UP(sl) ? R : RM;
+      for (int I = 0; I < NT; ++I) {
+        if (16 * I + 15 < k0 + pn || 16 * I >= n) continue;  // rows above or past the panel's reach
+        if (I > J && !lower_live) continue;
+        const int i = 16 * I + li;
+        const bool below = i >= k0 + pn && i < n;
 #pragma unroll
-      for (int q0 = 0; q0 < PB; q0 += 4) {
-        const int q = q0 + lk;
-        const double a = (below && q < pn) ? -R[q][i] * dinv_p[q] : 0.0;
-        acc[sl] = mfma16(a, E[q][x], acc[sl]);
+        for (int q0 = 0; q0 < PB; q0 += 4) {
+          const int q = q0 + lk;
+          const double a = (below && q < pn) ? -R[q][i] * dinv_p[q] : 0.0;
+          if (I < J) acc[I] = mfma16(a, R[q][xj], acc[I]);
+          else if (I > J) acc[I] = mfma16(a, RM[q][xj], acc[I]);
+          else {
+            acc[I] = mfma16(a, R[q][xj], acc[I]);
+            if (lower_live) accd = mfma16(a, RM[q][xj], accd);
+          }
+        }
       }
-    }
-    // the panel's rows take their eliminated values
+      // the panel's rows take their eliminated values
 #pragma unroll
-    for (int sl = 0; sl < SL; ++sl) {
-      if (!TILE_LIVE(sl) || TILE_I(sl) != k0 / 16) continue;
+      for (int I = 0; I < NT; ++I) {
+        if (I != k0 / 16) continue;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = 16 * TILE_I(sl) + 4 * v + lk, x = 16 * TILE_J(sl) + li, p = i - k0;
-        if (p >= 0 && p < pn && x < n) acc[sl][v] = R[p][x];
+        for (int v = 0; v < 4; ++v) {
+          const int i = 16 * I + 4 * v + lk, p = i - k0;
+          if (p >= 0 && p < pn && xj < n) {
+            acc[I][v] = R[p][xj];
+            if (I == J) accd[v] = R[p][xj];
+          }
+        }
       }
     }
     if (k0 + PB < n) publish(k0 + PB);
     __syncthreads();
   }
+  if (jlive) {
 #pragma unroll
-  for (int sl = 0; sl < SL; ++sl) {
-    if (!TILE_LIVE(sl) || !TILE_UP(sl) || TILE_I(sl) != TILE_J(sl)) continue;
+    for (int I = 0; I < NT; ++I) {
+      if (I != J) continue;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int i = 16 * TILE_I(sl) + 4 * v + lk, x = 16 * TILE_J(sl) + li;
-      if (x == i && i < n) dinv[i] = 1.0 / sqrt(acc[sl][v]);
+      for (int v = 0; v < 4; ++v) {
+        const int i = 16 * I + 4 * v + lk;
+        if (xj == i && i < n) dinv[i] = 1.0 / sqrt(acc[I][v]);
+      }
     }
   }
   __syncthreads();
+  if (jlive && xj < n) {
 #pragma unroll
-  for (int sl = 0; sl < SL; ++sl) {
-    if (!TILE_LIVE(sl)) continue;
+    for (int I = 0; I < NT; ++I)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int c = 16 * TILE_I(sl) + 4 * v + lk, r = 16 * TILE_J(sl) + li;  // X(r, c) from element (c, r)
-      if (c >= n || r >= n) continue;
-      if (TILE_UP(sl) && r >= c) X[r + (size_t)c * ldx] = r == c ? dinv[c] : 0.0;
-      if (!TILE_UP(sl) && r < c) X[r + (size_t)c * ldx] = acc[sl][v] * dinv[c];
-    }
+      for (int v = 0; v < 4; ++v) {
+        const int c = 16 * I + 4 * v + lk, r = xj;  // X(r, c) from element (c, r)
+        if (c >= n) continue;
+        double o;
+        if (r > c) o = 0.0;
+        else if (r == c) o = dinv[c];
+        else o = (I == J ? accd[v] : acc[I][v]) * dinv[c];
+        X[r + (size_t)c * ldx] = o;
+      }
   }
   const int zr = nz - n;
-  for (int e = tid; e < zr * n; e += 64 * NW) X[n + e % zr + (size_t)(e / zr) * ldx] = 0.0;
+  for (int e = tid; e < zr * n; e += 64 * NT) X[n + e % zr + (size_t)(e / zr) * ldx] = 0.0;
 }
-#undef TILE_I
-#undef TILE_J
-#undef TILE_UP
-#undef TILE_LIVE
 
 // C = alpha op(A) op(B) + beta C for the small band-block products of the
 // blocked Cholesky (FP64, 16 x 16 output tiles, K in LDS chunks of 16);
