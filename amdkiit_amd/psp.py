@@ -7,7 +7,10 @@ Produces, per species, everything the device path consumes:
   D        projector coupling matrix (diagonal KB weights for the CPMD
            numeric format; GTH h_ij blocks otherwise)
 
-Two sources:
+Three sources:
+* UPF v2 files (UpfSpecies), restating readupf.f90, form_factor_upf
+  (exp_igr.f90:142-209) and upf_nlppgrid/upf_nlpro
+  (pseudopotential.f90:234-354).
 * CPMD "NORMCONSERVING NUMERIC" files (the reference's own test inputs),
   restating readpsp.f90:11-67, exp_igr.f90:218-288 (form_factor),
   pseudopotential.f90:434-529 (nlppgrid), :108-162 (nlpro) with the same
@@ -345,6 +348,136 @@ class GTHSpecies(Species):
             return NaturalSpline(grid, tab)(g2)
 
         return f
+
+    def projectors(self):
+        return self._proj
+
+    def dmat(self):
+        return self._D
+
+
+def read_upf(path):
+    """UPF v2 file (the fields readupf.f90:19-425 reads): PP_HEADER
+    attributes, PP_R, PP_RAB, PP_LOCAL (Ry), PP_BETA.k with their
+    angular_momentum (r * beta(r), Ry^(1/2) units), PP_DIJ (Ry).  Parsed as
+    text, as the reference does, not as XML (PP_INFO is free text)."""
+    import re
+    with open(path) as f:
+        txt = f.read()
+
+    def attrs(tag_text):
+        return {k: v.strip() for k, v in re.findall(r'(\w+)\s*=\s*"([^"]*)"', tag_text)}
+
+    m = re.search(r"<PP_HEADER(.*?)/>", txt, re.S)
+    if m is None:
+        raise ValueError(f"{path}: no PP_HEADER (only UPF v2 is read)")
+    head = attrs(m.group(1))
+
+    def array(tag):
+        mm = re.search(r"<" + re.escape(tag) + r"\b([^>]*)>(.*?)</" + re.escape(tag) + r">", txt, re.S)
+        if mm is None:
+            raise ValueError(f"{path}: no {tag}")
+        return attrs(mm.group(1)), np.array(mm.group(2).split(), np.float64)
+
+    mesh = int(head["mesh_size"])
+    r = array("PP_R")[1][:mesh]
+    rab = array("PP_RAB")[1][:mesh]
+    vloc = array("PP_LOCAL")[1][:mesh]
+    nbeta = int(head["number_of_proj"])
+    betas = []
+    for k in range(1, nbeta + 1):
+        a, v = array(f"PP_BETA.{k}")
+        betas.append((int(a["angular_momentum"]), v[:mesh]))
+    dij = array("PP_DIJ")[1][:nbeta * nbeta].reshape(nbeta, nbeta) if nbeta else np.zeros((0, 0))
+    return dict(zv=float(head["z_valence"]), lmax=int(head["l_max"]), mesh=mesh, r=r, rab=rab, vloc=vloc,
+                betas=betas, dij=dij)
+
+
+class UpfSpecies(Species):
+    """Norm-conserving UPF v2 species as the reference tabulates it.
+
+    * Local part (exp_igr.f90:142-209, form_factor_upf): V_loc = PP_LOCAL / 2
+      (Ry -> Ha, readupf.f90:420), 4 pi SIMPSN of (r^2 V + Z erf(r / 1.2) r)
+      sin(Gr)/(Gr) with the PP_RAB weights, on the 5000-point G^2 grid.
+    * Projector list (readupf.f90:296-309): iv runs over l, then m, then the
+      projectors of that l; lp is the Y_lm index, lf the projector number.
+    * Radial tables (pseudopotential.f90:234-291, upf_nlppgrid): for each new
+      l the table is 4 pi SIMPSN(rab r beta j_l(Gr)) computed from the file's
+      beta function number l + 1 (1-based, in file order), and every further
+      projector of the same l copies the table of the first.  For files with
+      several projectors per l this is not the projector's own radial
+      function; it is restated as written because the recorded energies
+      come from it.
+    * D (readupf.f90:319-347): hlsg = PP_DIJ / 2 (Ry -> Ha) on the diagonal,
+      coupling projectors of the same (l, m); off-diagonal PP_DIJ must vanish.
+    """
+
+    def __init__(self, path, symbol, g2max_pw, g2max_rho):
+        d = read_upf(path)
+        self.symbol = symbol
+        self.zv = d["zv"]
+        r, rab, m = d["r"], d["rab"], d["mesh"]
+        sw = simpsn_weights(m)
+        vr = 0.5 * d["vloc"]
+        fint = r * r * vr + self.zv * erf(r / RAGGIO) * r
+        ggnh = np.arange(NSPLN) * (g2max_rho / (NSPLN - 1))
+        xg = np.sqrt(ggnh)
+        xrg = np.outer(xg, r)
+        big = xg > 1e-6
+        kern = np.ones_like(xrg)
+        kern[big] = np.sin(xrg[big]) / xrg[big]
+        voo = FOURPI * (kern * (fint * rab * sw)).sum(1)
+        self._loc = NaturalSpline(ggnh, voo)
+        # projector bookkeeping
+        nbeta = len(d["betas"])
+        npro = {}
+        pplist = []  # (l, index within l), 1-based index as the reference
+        for l, _ in d["betas"]:
+            npro[l] = npro.get(l, 0) + 1
+            if npro[l] > 4:
+                raise ValueError(f"{path}: more than 4 projectors for l = {l}")
+            pplist.append((l, npro[l]))
+        dij = d["dij"]
+        for i in range(nbeta):
+            for j in range(nbeta):
+                if i != j and dij[i, j] != 0.0:
+                    raise ValueError(f"{path}: non-diagonal PP_DIJ is not supported (readupf.f90:335-340)")
+        hlsg = {}
+        for i, (l, k) in enumerate(pplist):
+            hlsg[(l, k, k)] = 0.5 * dij[i, i]
+        lmax1 = d["lmax"] + 1
+        ivs = []  # (l, lp, lf)
+        lp = 0
+        for L in range(1, lmax1 + 1):
+            for _ in range(2 * L - 1):
+                lp += 1
+                for k in range(1, npro.get(L - 1, 0) + 1):
+                    ivs.append((L - 1, lp, k))
+        ggng = np.arange(NSPLN) * (g2max_pw / (NSPLN - 1))
+        xg = np.sqrt(ggng)
+        self._proj = []
+        lold, spl = -1, None
+        for (l, lpv, k) in ivs:
+            if l != lold:
+                if l >= nbeta:
+                    raise ValueError(f"{path}: no beta function number {l + 1} for l = {l}")
+                gnl = d["betas"][l][1]  # file's beta number l + 1 (see class note)
+                jl = sph_bessel(l, np.outer(xg, r).reshape(-1)).reshape(NSPLN, m)
+                tab = FOURPI * (jl * (rab * r * gnl * sw)).sum(1)
+                spl = NaturalSpline(ggng, tab)
+            lold = l
+            self._proj.append((l, lpv, spl))
+        n = len(ivs)
+        D = np.zeros((n, n))
+        for a, (l, lpa, ka) in enumerate(ivs):
+            for b, (l2, lpb, kb) in enumerate(ivs):
+                if l == l2 and lpa == lpb:
+                    D[a, b] = hlsg.get((l, ka, kb), 0.0) if ka == kb else 0.0
+        self._D = D
+        self.ivs = ivs
+
+    def local(self, g2):
+        return self._loc(g2)
 
     def projectors(self):
         return self._proj

@@ -12,12 +12,15 @@ GOLDEN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))
 
 
 def read_xyz(path):
+    """Coordinates as readcoordinate (yamlread.f90:358-424) reads them: two
+    header lines skipped, then one atom per line to the end of the file."""
     with open(path) as f:
         lines = f.read().split("\n")
-    n = int(lines[0].split()[0])
     sym, pos = [], []
-    for ln in lines[2:2 + n]:
+    for ln in lines[2:]:
         t = ln.split()
+        if len(t) < 4:
+            continue
         sym.append(t[0])
         pos.append([float(x) for x in t[1:4]])
     return sym, np.array(pos)
@@ -47,6 +50,26 @@ def reference_water(nmol, ecut=80.0, box_ang=9.85):
     nel = 6 * nmol + 2 * nmol
     nb = nel // 2
     return System(pw, [sO, sH], sp, tau, nb, np.full(nb, 2.0), "TETER93", f"h2o_{nmol}_ref")
+
+
+def reference_h2_upf():
+    """Recorded UPF run (tests/H2_upf): H2 in a 15.1178 bohr cubic cell,
+    ONCV H UPF v2 (input LMAX S, LOCAL 0 are not read for UPF species),
+    CUTOFF absent -> 80 Ry (yamlread.f90:132-138), XC_LDA_XC_TETER93,
+    coordinates in bohr, one doubly occupied state."""
+    sym, pos = read_xyz(os.path.join(GOLDEN, "structures", "h2_upf.xyz"))
+    pw = PWSetup(Cell.cubic(15.1178), 80.0)
+    sH = psp.UpfSpecies(os.path.join(GOLDEN, "psp", "H_ONCV.upf"), "H", pw.ecut, pw.ecut_rho)
+    return System(pw, [sH], [0] * len(sym), pos, 1, np.array([2.0]), "TETER93", "h2_upf_ref")
+
+
+def h2_upf_small(ecut=30.0, nbands=4):
+    """The tests/H2_upf structure and UPF species at a lower cutoff with
+    several occupied bands: a small case for per-function parity tests."""
+    sym, pos = read_xyz(os.path.join(GOLDEN, "structures", "h2_upf.xyz"))
+    pw = PWSetup(Cell.cubic(15.1178), ecut)
+    sH = psp.UpfSpecies(os.path.join(GOLDEN, "psp", "H_ONCV.upf"), "H", pw.ecut, pw.ecut_rho)
+    return System(pw, [sH], [0] * len(sym), pos, nbands, np.full(nbands, 2.0), "PZ", "h2_upf_small")
 
 
 class SplitMix64:

@@ -114,10 +114,12 @@ int amdk_set_positions(amdk_ctx* ctx, int nat, const int* sp_of_atom, const doub
 int amdk_nproj(const amdk_ctx* ctx);
 
 /* nl(p, b) = <beta_p|psi_b> (full sphere), column-major (nproj, nb).
- * Replaces fnl(C0, nstat) (pseudopotential.f90:164-226). */
+ * Replaces fnl(C0, nstat) (pseudopotential.f90:164-226) and upf_fnl
+ * (pseudopotential.f90:356-433): the same product with UPF tables. */
 int amdk_fnl(amdk_ctx* ctx, const double* c0, int ldc, int nb, double* nl);
 /* c2 += -f~ sum_p beta_p (D nl)_p with f~ = f (>= 1e-5) else 1.
- * Replaces FNONLOC(CA, NOSTATE, FC) (wfn_initialize.f90:560-612). */
+ * Replaces FNONLOC(CA, NOSTATE, FC) (wfn_initialize.f90:560-612) and
+ * UPF_FNONLOC (wfn_initialize.f90:614-695; D = hlsg over the same (l, m)). */
 int amdk_fnonloc(amdk_ctx* ctx, const double* nl, double* c2, int ldc, const double* occ, int nb);
 /* A(i,j) = 2 Re <ca_i|cb_j>_half - ca_i(0) cb_j(0), column-major (n1, n2).
  * Replaces OVLAP_D(NG_PW, N1, N2, A, CA, CB) (wfn_initialize.f90:234-263). */
@@ -146,8 +148,9 @@ typedef struct {
  * e receives eh, eloc, exc (others untouched). */
 int amdk_vks(amdk_ctx* ctx, const double* rho_r, int xc, double* v_r, amdk_energies* e);
 
-/* Forces -dE/dR (Ha/bohr) [3*nat]: POTFOR + NLFOR/D_FNL + E_SR
- * (nuclear_grad.f90:4-372), using rho(G) of the last amdk_vks call. */
+/* Forces -dE/dR (Ha/bohr) [3*nat]: POTFOR + NLFOR/D_FNL (or UPF_NLFOR/
+ * UPF_D_FNL, nuclear_grad.f90:90-199) + E_SR (nuclear_grad.f90:4-372),
+ * using rho(G) of the last amdk_vks call. */
 int amdk_forces(amdk_ctx* ctx, const double* c0, int ldc, const double* occ, int nb, double* f);
 
 /* Device-resident SCF (wfn_opt loop body, do_opt.f90:80-129): load C_0,

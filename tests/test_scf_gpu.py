@@ -33,6 +33,7 @@ SMALL = {
     "h2o2_pbe": lambda: systems.water_box((1, 1, 2), 10.0, 20.0, "PBE"),
     "h2o2_pz": lambda: systems.water_box((1, 1, 2), 10.0, 20.0, "PZ"),
     "si8_lda": lambda: systems.si64_like(cells=1, ecut=12.0),
+    "h2_upf": lambda: systems.h2_upf_small(),
 }
 TOL = {"fp64": 1e-11, "mixed": 3e-5}
 
@@ -206,6 +207,32 @@ def test_scf_reproduces_recorded_reference_energy(nmol, prec):
     assert abs(e["esr"] - ref["esr"]) < 1e-9
     assert abs(e["eself"] - ref["eself"]) < 1e-9
     assert abs(e["nelec"] - s.occ.sum()) < 1e-10 * s.occ.sum()
+    ctx.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_scf_reproduces_recorded_upf_run(prec):
+    """tests/H2_upf (UPF v2 ONCV hydrogen, 5 projectors per atom, one band):
+    converged E_tot against the recorded -1.024460 (F16.6), FP64 within the
+    print rounding, mixed within a further 1e-6 Ha/atom; the GPU nonlocal
+    terms also against the oracle on the converged wavefunction."""
+    ref = REF["h2_upf"]
+    s = systems.reference_h2_upf()
+    print(f"case=h2_upf prec={prec} n={s.pw.n} npw={s.pw.ngpw} nb={s.nbands}", flush=True)
+    ctx = KSContext(s.pw, s.nbands, prec)
+    ctx.set_species(s.tables)
+    ctx.set_positions(s.sp, s.tau)
+    orc = KSOracle(s.pw, s.tables, s.sp, s.tau, s.xc)
+    c0 = orc.ortho(random_bands(s.pw, s.nbands, seed=1))
+    e, it = run_scf(ctx, s, c0, 1e-7 if prec == "fp64" else 1e-6)
+    print(f"  steps={it} etot={e['etot']:.10f} ref={ref['etot']:.6f} enl={e['enl']:.10f}", flush=True)
+    tol = ref["etot_halfwidth"] + (0.0 if prec == "fp64" else 1e-6 * s.nat)
+    assert abs(e["etot"] - ref["etot"]) < tol
+    assert abs(e["nelec"] - s.occ.sum()) < 1e-10 * s.occ.sum()
+    # the stored C_0 has taken this (converged) step's update since its
+    # energies were evaluated, so E_nl agrees to the convergence level only
+    c = ctx.wfn_store()
+    assert abs(e["enl"] - orc.e_nl(orc.fnl(c), s.occ)) < 1e-6
     ctx.close()
 
 
