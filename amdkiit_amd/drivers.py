@@ -1,12 +1,14 @@
 """Host drivers around the SCF: Born-Oppenheimer MD and L-BFGS geometry
 optimisation (the reference's molecular_dynamics.f90 / update_velver.f90 and
-do_opt.f90:197-266 + lbfgs.f), driven by the device-resident SCF of libamdk.
+do_opt.f90:197-266 + lbfgs.f), driven by the device-resident SCF of libamdk,
+writing the reference's output files when given a RunOutputs.
 
 The drivers only need a force provider `provider(tau) -> (E, F)` (Hartree,
 Ha/bohr, F = -dE/dR), so they are tested on the CPU with analytic potentials
 and on the GPU with `GPUForceProvider`.
 """
 import math
+import time
 
 import numpy as np
 
@@ -24,8 +26,9 @@ class GPUForceProvider:
     wavefunction is the starting guess, forces are taken in the converged
     step before its wavefunction update, as TFOR does in do_opt.f90:97-106)."""
 
-    def __init__(self, ctx, system, c0, gcut=1e-6, maxit=2000):
+    def __init__(self, ctx, system, c0, gcut=1e-6, maxit=2000, outputs=None):
         self.ctx, self.s, self.gcut, self.maxit = ctx, system, gcut, maxit
+        self.outputs = outputs  # RunOutputs: SP_ENERGY.dat lines per SCF step
         self.ctx.wfn_load(c0, system.occ)
         self.nscf = []
 
@@ -37,7 +40,10 @@ class GPUForceProvider:
         f = None
         for it in range(1, self.maxit + 1):
             want = prev < 3.0 * self.gcut
+            t0 = time.perf_counter()
             e, fi = self.ctx.scf_step_tfor(it, s.xc, want)
+            if self.outputs is not None:
+                self.outputs.sp_energy(it, e["etot"], 1e3 * (time.perf_counter() - t0))
             if fi is not None:
                 f = fi
             prev = e["gemax"]
@@ -66,18 +72,35 @@ def kinetic(masses_au, v):
     return 0.5 * float((masses_au[:, None] * v * v).sum())
 
 
-def bomd(provider, tau0, masses_au, v0, dt_fs, nsteps, log=None):
+def temperature(masses_au, v):
+    """Instantaneous temperature with ndof = 3N - 3 (temp_ke, md_tools.f90:50-67;
+    molecular_dynamics.f90:39)."""
+    return 2.0 * kinetic(masses_au, v) / (KBOLTZ * (3 * len(v) - 3))
+
+
+def bomd(provider, tau0, masses_au, v0, dt_fs, nsteps, log=None, outputs=None, symbols=None):
     """NVE velocity Verlet (velup_velver / posup_velver, update_velver.f90:14-48)
     with the centre of mass held fixed (shift_com).  Returns per-step
-    (E_pot, E_kin, E_cons) and the drift of E_cons in Ha/atom/ps."""
+    (E_pot, E_kin, E_cons) and the drift of E_cons in Ha/atom/ps.  With
+    `outputs` (RunOutputs) and `symbols`, MD_TRAJECTORY.xyz and MD_ENERGY.dat
+    get one frame per step from step 0, as molecular_dynamics.f90:74-127."""
     dt = dt_fs / AU_TO_FS
     tau = np.array(tau0, np.float64)
     v = np.array(v0, np.float64)
     m = np.asarray(masses_au, np.float64)
     com0 = (m[:, None] * tau).sum(0) / m.sum()
+
+    def record(step, e, ek, t0):
+        if outputs is not None:
+            outputs.md_trajectory(step, symbols, tau, v)
+            outputs.md_energy(step, temperature(m, v), e, e + ek, time.perf_counter() - t0)
+
+    t0 = time.perf_counter()
     e, f = provider(tau)
     hist = [(e, kinetic(m, v), e + kinetic(m, v))]
+    record(0, e, hist[0][1], t0)
     for step in range(1, nsteps + 1):
+        t0 = time.perf_counter()
         v += 0.5 * dt * f / m[:, None]
         tau += dt * v
         tau -= (m[:, None] * tau).sum(0) / m.sum() - com0
@@ -85,6 +108,7 @@ def bomd(provider, tau0, masses_au, v0, dt_fs, nsteps, log=None):
         v += 0.5 * dt * f / m[:, None]
         ek = kinetic(m, v)
         hist.append((e, ek, e + ek))
+        record(step, e, ek, t0)
         if log:
             log(step, e, ek, e + ek)
     h = np.array(hist)
@@ -93,11 +117,15 @@ def bomd(provider, tau0, masses_au, v0, dt_fs, nsteps, log=None):
     return h, drift, tau, v
 
 
-def lbfgs(provider, x0, m=5, gtol=1e-4, maxiter=20, log=None):
+def lbfgs(provider, x0, m=5, gtol=1e-4, maxiter=20, log=None, outputs=None, symbols=None):
     """L-BFGS on E(R) with the two-loop recursion and M = 5 corrections, as the
     reference's geom_opt_lbfgs (do_opt.f90:197-266) calls lbfgs.f.  The step
     length comes from a strong-Wolfe line search (c1 = 1e-4, c2 = 0.9, Nocedal &
-    Wright alg. 3.5/3.6) instead of lbfgs.f's Moré-Thuente routine."""
+    Wright alg. 3.5/3.6) instead of lbfgs.f's Moré-Thuente routine.  With
+    `outputs` and `symbols`, every accepted geometry (the first included) is
+    appended to atom_coord.xyz and atom_force.dat, the latter holding the
+    gradient dE/dR as the reference's PRINT_COORDINATE call does
+    (do_opt.f90:244-249)."""
     x = np.array(x0, np.float64).reshape(-1)
     shape = np.shape(x0)
 
@@ -105,9 +133,15 @@ def lbfgs(provider, x0, m=5, gtol=1e-4, maxiter=20, log=None):
         e, f = provider(xx.reshape(shape))
         return e, -np.asarray(f, np.float64).reshape(-1)
 
+    def record(iopt, e, xx, g):
+        if outputs is not None:
+            outputs.atom_coord(iopt, e, symbols, xx.reshape(-1, 3))
+            outputs.atom_force(iopt, e, symbols, g.reshape(-1, 3))
+
     e, g = fg(x)
     S, Y = [], []
     hist = [(e, float(np.abs(g).max()))]
+    record(0, e, x, g)
     for it in range(1, maxiter + 1):
         if np.abs(g).max() < gtol:
             break
@@ -140,6 +174,7 @@ def lbfgs(provider, x0, m=5, gtol=1e-4, maxiter=20, log=None):
                 S.pop(0)
                 Y.pop(0)
         hist.append((e, float(np.abs(g).max())))
+        record(it, e, x, g)
         if log:
             log(it, e, float(np.abs(g).max()))
     return x.reshape(shape), hist

@@ -78,3 +78,37 @@ def test_lbfgs_minimises_quadratic_and_rosenbrock():
     x, hist = D.lbfgs(rosen, np.array([-1.2, 1.0]), gtol=1e-6, maxiter=200)
     assert np.abs(x - 1.0).max() < 1e-4
     assert all(hist[i + 1][0] <= hist[i][0] + 1e-12 for i in range(len(hist) - 1))
+
+
+def test_drivers_write_the_reference_output_files(tmp_path):
+    """MD writes MD_TRAJECTORY.xyz and MD_ENERGY.dat frames from step 0, the
+    geometry optimiser atom_coord.xyz and atom_force.dat (gradient) per
+    accepted geometry; the files read back to the driver's own numbers."""
+    from amdkiit_amd.outputs import ANGS2BOHR, RunOutputs
+    rng = np.random.default_rng(3)
+    x0 = rng.normal(size=(3, 3)) * 1.5
+    m = np.array([16.0, 1.0, 1.0]) * D.AMU_AU
+    prov = spring_provider(0.3, x0)
+    v = D.maxwell_boltzmann(m, 300.0, seed=2)
+    out = RunOutputs(str(tmp_path / "md"))
+    h, _, tau, v1 = D.bomd(prov, x0, m, v, dt_fs=0.2, nsteps=5, outputs=out, symbols=["O", "H", "H"])
+    en = [ln.split() for ln in (tmp_path / "md" / "MD_ENERGY.dat").read_text().splitlines()]
+    assert [int(r[0]) for r in en] == list(range(6))
+    assert abs(float(en[0][1]) - 300.0) < 1e-9  # temperature with ndof = 3N - 3
+    assert np.allclose([float(r[2]) for r in en], h[:, 0], rtol=0, atol=1e-14)
+    assert np.allclose([float(r[3]) for r in en], h[:, 2], rtol=0, atol=1e-14)
+    tr = (tmp_path / "md" / "MD_TRAJECTORY.xyz").read_text().splitlines()
+    assert len(tr) == 6 * 5 and tr[1].split() == ["MD", "STEP=", "0"] and tr[-4].split()[-1] == "5"
+    last = np.array([[float(x) for x in ln[2:].split()] for ln in tr[-3:]])
+    assert np.abs(last[:, :3] * ANGS2BOHR - tau).max() < 1e-5
+    assert np.abs(last[:, 3:] * ANGS2BOHR - v1).max() < 1e-5
+    out = RunOutputs(str(tmp_path / "opt"))
+    x, hist = D.lbfgs(prov, x0, gtol=1e-6, maxiter=30, outputs=out, symbols=["O", "H", "H"])
+    fr = (tmp_path / "opt" / "atom_force.dat").read_text().splitlines()
+    assert len(fr) == 5 * len(hist)
+    e_last = float(fr[-4].split()[1])
+    g_last = np.array([[float(v) for v in ln.split()[1:]] for ln in fr[-3:]])
+    e, f = prov(x)
+    assert abs(e_last - e) < 1e-14 and np.abs(g_last + f).max() < 1e-14  # gradient = -F
+    co = (tmp_path / "opt" / "atom_coord.xyz").read_text().splitlines()
+    assert np.abs(np.array([[float(v) for v in ln.split()[1:]] for ln in co[-3:]]) - x * 0.529177).max() < 1e-14
