@@ -80,7 +80,7 @@ def kernel_bytes(ctx, pw, nb, prec):
     }
 
 
-def run_gpu(sysobj, prec, steps, warmup, c0, profile_steps=3):
+def run_gpu(sysobj, prec, steps, warmup, c0, profile_steps=3, tfor_steps=0):
     import torch
 
     from amdkiit_amd.context import KSContext
@@ -114,7 +114,33 @@ def run_gpu(sysobj, prec, steps, warmup, c0, profile_steps=3):
     for name in ctx.profile_names():
         tot, cnt = ctx.profile_query(name)
         prof[name] = (tot, cnt)
+    if tfor_steps:
+        # forces-on steps (the TFOR branch: E_SR + POTFOR + NLFOR inside the step)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(tfor_steps):
+            it += 1
+            ctx.scf_step_tfor(it, s.xc, True)
+        torch.cuda.synchronize()
+        prof["_tfor_ms_per_step"] = ((time.perf_counter() - t1) / tfor_steps * 1e3, tfor_steps)
     return ctx, dt, e, prof, it
+
+
+def copy_ceiling(nbytes=1 << 30, reps=10):
+    """Measured device-to-device copy rate (read + write bytes / time)."""
+    import torch
+
+    x = torch.empty(nbytes, dtype=torch.uint8, device="cuda").fill_(1)
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    gbs = 2.0 * nbytes * reps / (time.perf_counter() - t0) / 1e9
+    del x, y
+    return gbs
 
 
 MFMA_PEAK = {"f64": 78.6, "f32": 157.3}  # TFLOP/s, MI355X dense
@@ -266,10 +292,14 @@ def main():
     c0[:, 0] = c0[:, 0].real
     log(f"[bench] n={s.pw.n} ngpw={s.pw.ngpw} nb={s.nbands} nat={s.nat}")
 
-    ctx, dt, e, prof, it = run_gpu(s, a.precision, a.steps, a.warmup, c0)
+    ctx, dt, e, prof, it = run_gpu(s, a.precision, a.steps, a.warmup, c0, tfor_steps=min(a.steps, 10))
+    tfor_ms = prof.pop("_tfor_ms_per_step")[0]
     ms = dt / a.steps * 1e3
     value = a.steps / dt
-    log(f"[bench] {a.precision}: {ms:.3f} ms/step  E={e['etot']:.8f}  gemax={e['gemax']:.3e}")
+    log(f"[bench] {a.precision}: {ms:.3f} ms/step  E={e['etot']:.8f}  gemax={e['gemax']:.3e}; "
+        f"forces-on step {tfor_ms:.3f} ms")
+    copy_gbs = copy_ceiling()
+    log(f"[bench] device copy ceiling {copy_gbs:.0f} GB/s")
 
     kb = kernel_bytes(ctx, s.pw, s.nbands, a.precision)
     hp_ms = sum(prof[k][0] / prof[k][1] for k in HPSI_KERNELS if k in prof and prof[k][1])
@@ -368,7 +398,10 @@ def main():
         "config": {"workload": CONFIGS[a.config], "precision": a.precision, "grid": list(s.pw.n),
                    "ngpw": s.pw.ngpw, "bands": s.nbands, "atoms": s.nat, "xc": s.xc},
         "hpsi": {"ms": hp_ms, "alg_bytes": hb, "GBps": hp_gbs, "frac_hbm": hp_gbs / HBM_PEAK,
-                 "launches": list(HPSI_KERNELS)},
+                 "frac_copy_ceiling": hp_gbs / copy_gbs, "launches": list(HPSI_KERNELS)},
+        "forces_on_ms_per_step": tfor_ms,
+        "forces_off_ms_per_step": ms,
+        "hbm_copy_ceiling_GBps": copy_gbs,
         "roofline": {"bound": "hbm", "kernel": "hpsi_local (" + "+".join(HPSI_KERNELS) + ")",
                      "achieved": hp_gbs, "peak": HBM_PEAK, "unit": "GB/s", "frac": hp_gbs / HBM_PEAK,
                      "traffic": traffic["hpsi_local_bytes"] if traffic else None,
