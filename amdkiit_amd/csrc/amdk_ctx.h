@@ -121,7 +121,8 @@ struct amdk_ctx {
   double* d_wnorm = nullptr;  // 1 for occupied bands (norm weights)
   double* d_wdens = nullptr;  // f_b for the density norm
   double* d_pcg = nullptr;    // PCG scalars
-  int* d_info = nullptr;
+  int* d_info = nullptr;      // directly after d_red[16]
+  void* h_stage = nullptr;    // pinned host copy of d_red and d_info
   // resident wavefunction state for amdk_scf_step
   void* C0 = nullptr;
   void* C2 = nullptr;

@@ -123,11 +123,14 @@ __device__ __forceinline__ void gvec_of(const int* mill, int g, const Recip& rc,
 
 // Hartree + local: vg = 4 pi/G^2 (rho + rho_I) + V_ps (G != 0), vg(0) = 0
 // (potential.f90:4-46) and the partial sums of E_H and E_loc
-// (total_energy.f90:64-131, pseudopotential.f90:22-34).
+// (total_energy.f90:64-131, pseudopotential.f90:22-34).  With f0 non-null
+// (GGA) also the fields for grad rho: pair f0 = (i Gx rho, i Gy rho), pair
+// f1 = (i Gz rho, 0), zero on the padding up to ldc.
 __global__ void __launch_bounds__(256) k_hartree(const int* __restrict__ mill, int ng, Recip rc,
                                                  const double2* __restrict__ rhog, const double2* __restrict__ sr,
                                                  const double2* __restrict__ sv, double2* __restrict__ vg,
-                                                 double* __restrict__ part) {
+                                                 double* __restrict__ part, int ldc, double2* __restrict__ f0,
+                                                 double2* __restrict__ f1) {
   __shared__ double sh[4];
   double eh = 0, el = 0;
   for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x) {
@@ -135,6 +138,12 @@ __global__ void __launch_bounds__(256) k_hartree(const int* __restrict__ mill, i
     gvec_of(mill, g, rc, gx, gy, gz);
     const double g2 = gx * gx + gy * gy + gz * gz;
     const double2 r = rhog[g], ri = sr[g], v = sv[g];
+    if (f0) {
+      f0[g] = make_double2(-gx * r.y, gx * r.x);
+      f0[ldc + g] = make_double2(-gy * r.y, gy * r.x);
+      f1[g] = make_double2(-gz * r.y, gz * r.x);
+      f1[ldc + g] = make_double2(0, 0);
+    }
     if (g2 > 1e-12) {
       const double f = 4.0 * M_PI / g2;
       const double2 rt = make_double2(r.x + ri.x, r.y + ri.y);
@@ -146,6 +155,9 @@ __global__ void __launch_bounds__(256) k_hartree(const int* __restrict__ mill, i
       vg[g] = make_double2(0, 0);
     }
   }
+  if (f0)
+    for (int g = ng + blockIdx.x * blockDim.x + threadIdx.x; g < ldc; g += gridDim.x * blockDim.x)
+      f0[g] = f0[ldc + g] = f1[g] = f1[ldc + g] = make_double2(0, 0);
   const double a = block_sum(eh, sh);
   const double b = block_sum(el, sh);
   if (threadIdx.x == 0) {
@@ -154,24 +166,6 @@ __global__ void __launch_bounds__(256) k_hartree(const int* __restrict__ mill, i
   }
 }
 
-// fields for grad rho: pair 0 = (i Gx rho, i Gy rho), pair 1 = (i Gz rho, 0)
-__global__ void k_grad_fields(const int* __restrict__ mill, int ng, int ldc, Recip rc,
-                              const double2* __restrict__ rhog, double2* __restrict__ f0,
-                              double2* __restrict__ f1) {
-  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ldc; g += gridDim.x * blockDim.x) {
-    if (g < ng) {
-      double gx, gy, gz;
-      gvec_of(mill, g, rc, gx, gy, gz);
-      const double2 r = rhog[g];
-      f0[g] = make_double2(-gx * r.y, gx * r.x);
-      f0[ldc + g] = make_double2(-gy * r.y, gy * r.x);
-      f1[g] = make_double2(-gz * r.y, gz * r.x);
-      f1[ldc + g] = make_double2(0, 0);
-    } else {
-      f0[g] = f0[ldc + g] = f1[g] = f1[ldc + g] = make_double2(0, 0);
-    }
-  }
-}
 
 // vg += A1 - i Gx B1 - i Gy A2 - i Gz B2  (xc.f90:264-296); A2/B2 may be null (LDA)
 __global__ void k_vassemble(const int* __restrict__ mill, int ng, int ldc, Recip rc,
@@ -284,15 +278,16 @@ __device__ __forceinline__ void xc_pbe(double rho, double sigma, double& f, doub
 
 // XC on the grid with the reference's low-density smoothing (xc.f90:100-133).
 // In: rho, grad components (GGA only).  Out: v1 -> va; v2*grad -> gx,gy,gz in
-// place; partial E_xc sums (times Omega/N later).
+// place; partial E_xc sums (times Omega/N later) and grid sums of rho.
 __global__ void __launch_bounds__(256) k_xc(int kind, size_t nr, const double* __restrict__ rho,
                                             double* __restrict__ va, double* __restrict__ gx,
                                             double* __restrict__ gy, double* __restrict__ gz,
-                                            double* __restrict__ part) {
+                                            double* __restrict__ part, double* __restrict__ part_rho) {
   __shared__ double sh[4];
   const double gc = 1.0e-8;
-  double exc = 0;
+  double exc = 0, rsum = 0;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nr; i += (size_t)gridDim.x * blockDim.x) {
+    rsum += rho[i];
     const double r = fmax(rho[i], 0.0);
     double v1 = 0, v2 = 0;
     if (r > 0.1 * gc) {
@@ -321,7 +316,11 @@ __global__ void __launch_bounds__(256) k_xc(int kind, size_t nr, const double* _
     }
   }
   const double e = block_sum(exc, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = e;
+  const double rs = block_sum(rsum, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = e;
+    part_rho[blockIdx.x] = rs;
+  }
 }
 
 // sum of a real grid (for the electron count)
