@@ -314,13 +314,15 @@ static void reg_z_vpsi(amdk_ctx* c, const Sphere& S, int np) {
                                                             (const V*)(sizeof(T) == 4 ? c->tw_f[2] : c->tw_d[2]));
 }
 template <typename T, int A, int B>
-static void reg_z_rho(amdk_ctx* c, const Sphere& S, int np, double* rho, int accumulate, const double* wpair) {
+static void reg_z_rho(amdk_ctx* c, const Sphere& S, int np, double* rho, int accumulate, const double* wpair,
+                      int groups, size_t part_stride) {
   using V = typename cplx_of<T>::type;
   constexpr int N = A * B, TM = MaxAB<A, B>::v, TX = RegTiles<T, TM>::TX;
   const size_t lds = (size_t)(N + N * TX) * sizeof(V);
-  dim3 grid((c->n1 + TX - 1) / TX, c->n2);
+  dim3 grid((c->n1 + TX - 1) / TX, c->n2, groups);
   k_z_rho_r<T, A, B, TX><<<grid, TX * TM, lds, c->stream>>>(geom_of(c, S), (const V*)c->W2, rho, np, wpair, accumulate,
-                                                           (const V*)(sizeof(T) == 4 ? c->tw_f[2] : c->tw_d[2]));
+                                                           (const V*)(sizeof(T) == 4 ? c->tw_f[2] : c->tw_d[2]),
+                                                           part_stride);
 }
 template <typename T, int A, int B>
 static void reg_cols(amdk_ctx* c, const Sphere& S, int np, bool inverse) {
@@ -465,7 +467,10 @@ static void launch_rows_fwd_hpsi(amdk_ctx* c, const Sphere& S, const void* C, vo
 }
 
 template <typename T>
-static void launch_z_rho(amdk_ctx* c, const Sphere& S, int np, double* rho, int accumulate, const double* wpair) {
+// groups > 1 (register path only): pair groups accumulate into `groups` grids
+// spaced part_stride apart, starting at rho.
+static void launch_z_rho(amdk_ctx* c, const Sphere& S, int np, double* rho, int accumulate, const double* wpair,
+                         int groups = 1, size_t part_stride = 0) {
   using V = typename cplx_of<T>::type;
   constexpr int TX = Tiles<T>::TX;
   const size_t lds = (size_t)(c->n3 + 2 * c->n3 * TX) * sizeof(V);
@@ -473,7 +478,7 @@ static void launch_z_rho(amdk_ctx* c, const Sphere& S, int np, double* rho, int 
   const int ps = prof_begin(c, "z_rho");
   if (reg_ok(c, c->n3)) {
     switch (c->n3) {
-#define AMDK_C(N, A, B) case N: reg_z_rho<T, A, B>(c, S, np, rho, accumulate, wpair); break;
+#define AMDK_C(N, A, B) case N: reg_z_rho<T, A, B>(c, S, np, rho, accumulate, wpair, groups, part_stride); break;
       AMDK_REG_SIZES(AMDK_C)
 #undef AMDK_C
     }
@@ -481,6 +486,7 @@ static void launch_z_rho(amdk_ctx* c, const Sphere& S, int np, double* rho, int 
     prof_end(c, ps);
     return;
   }
+  if (groups != 1) throw Error(AMDK_ERR_STATE, "grouped density pass needs the register FFT path");
   k_z_rho<T, TX, 16><<<grid, 256, lds, c->stream>>>(geom_of(c, S), (const V*)c->W2, rho, np, wpair, accumulate, c->p3,
                                                      (const V*)(sizeof(T) == 4 ? c->tw_f[2] : c->tw_d[2]));
   AMDK_TRY_HIP(hipGetLastError());
@@ -521,17 +527,29 @@ static void prepare_density_weights(amdk_ctx* c, const double* occ, int nb) {
 }
 
 // rho(r) for bands [0, nb) with the weights of prepare_density_weights
+// Density of the pairs of C0 into rho.  Into grid[0] with many pairs, the z
+// pass runs in RHO_GROUPS pair groups (more blocks in flight) that accumulate
+// into grid[1 ..] and are summed into rho by k_rho_combine, which also leaves
+// the grid-sum partials in slot 8.
+static constexpr int RHO_GROUPS = 4;
 template <typename T>
 static void run_density(amdk_ctx* c, const void* C0, int nb, double* rho, bool norms_ready = false) {
   const Sphere& S = c->spw;
   const int npairs = (nb + 1) / 2;
+  const size_t nr = (size_t)c->n1 * c->n2 * c->n3;
+  const bool grouped = rho == c->grid[0] && reg_ok(c, c->n3) && std::min(npairs, c->cap_pairs) >= 2 * RHO_GROUPS;
   for (int p0 = 0; p0 < npairs; p0 += c->cap_pairs) {
     const int np = std::min(c->cap_pairs, npairs - p0);
     launch_rows_inv<T>(c, S, C0, S.ldc, nb, p0, np);
     launch_cols<T>(c, S, np, true);
-    launch_z_rho<T>(c, S, np, rho, p0 > 0, c->d_wpair + 2 * p0);
+    if (grouped) launch_z_rho<T>(c, S, np, c->grid[1], p0 > 0, c->d_wpair + 2 * p0, RHO_GROUPS, nr);
+    else launch_z_rho<T>(c, S, np, rho, p0 > 0, c->d_wpair + 2 * p0);
   }
-  if (sizeof(T) == 4) renormalise_density<T>(c, C0, nb, rho, norms_ready);
+  if (grouped) {
+    k_rho_combine<<<RED_BLOCKS, 256, 0, c->stream>>>(c->grid[1], nr, RHO_GROUPS, nr, rho, c->d_part + 8 * RED_BLOCKS);
+    AMDK_TRY_HIP(hipGetLastError());
+  }
+  if (sizeof(T) == 4) renormalise_density<T>(c, C0, nb, rho, norms_ready, grouped);
 }
 
 // ---------------------------------------------------------------------------

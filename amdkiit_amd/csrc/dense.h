@@ -323,6 +323,21 @@ __global__ void __launch_bounds__(256) k_xc(int kind, size_t nr, const double* _
   }
 }
 
+// rho = sum of G partial grids (fixed order) and the block sums of rho
+__global__ void __launch_bounds__(256) k_rho_combine(const double* __restrict__ part, size_t stride, int G, size_t n,
+                                                     double* __restrict__ rho, double* __restrict__ sums) {
+  __shared__ double sh[4];
+  double s = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    double v = part[i];
+    for (int g = 1; g < G; ++g) v += part[g * stride + i];
+    rho[i] = v;
+    s += v;
+  }
+  const double r = block_sum(s, sh);
+  if (threadIdx.x == 0) sums[blockIdx.x] = r;
+}
+
 // sum of a real grid (for the electron count)
 __global__ void __launch_bounds__(256) k_grid_sum(const double* __restrict__ x, size_t n, double* __restrict__ part) {
   __shared__ double sh[4];

@@ -176,11 +176,14 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
 }
 
 // z pass of the density: z-FFT exp(+i) of every pair, rho += w1 Re^2 + w2 Im^2
-// in FP64 registers (each thread owns fixed z), ordered over pairs.
+// in FP64 registers (each thread owns fixed z), ordered over pairs.  The
+// pairs are split into gridDim.z consecutive groups; group k accumulates
+// into the grid rho + k * part_stride (summed afterwards in a fixed order).
 template <typename T, int A, int B, int TX>
 __global__ void __launch_bounds__(TX * (A > B ? A : B))
     k_z_rho_r(PassGeom g, const typename cplx_of<T>::type* __restrict__ W2, double* __restrict__ rho, int npairs,
-              const double* __restrict__ wpair, int accumulate, const typename cplx_of<T>::type* __restrict__ twg) {
+              const double* __restrict__ wpair, int accumulate, const typename cplx_of<T>::type* __restrict__ twg,
+              size_t part_stride) {
   using V = typename cplx_of<T>::type;
   constexpr int N = A * B, TM = MaxAB<A, B>::v;
   extern __shared__ unsigned char smem_raw[];
@@ -211,15 +214,18 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
       else { dst[k2].x = 0; dst[k2].y = 0; }
     }
   };
-  if (PF && npairs > 0) load_pair(0, rn);
-  for (int pl = 0; pl < npairs; ++pl) {
+  const int per = (npairs + gridDim.z - 1) / gridDim.z;
+  const int pa = blockIdx.z * per, pb = min(npairs, pa + per);
+  rho += blockIdx.z * part_stride;
+  if (PF && pa < pb) load_pair(pa, rn);
+  for (int pl = pa; pl < pb; ++pl) {
     V r[TM];
 #pragma unroll
     for (int k = 0; k < TM; ++k) { r[k].x = 0; r[k].y = 0; }
     if constexpr (PF) {
 #pragma unroll
       for (int k2 = 0; k2 < B; ++k2) r[k2] = rn[k2];
-      if (pl + 1 < npairs) load_pair(pl + 1, rn);
+      if (pl + 1 < pb) load_pair(pl + 1, rn);
     } else {
       load_pair(pl, r);
     }
