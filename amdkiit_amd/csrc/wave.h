@@ -32,7 +32,7 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
   return r;
 }
 
-constexpr int RED_BLOCKS = 512;
+constexpr int RED_BLOCKS = 1024;
 
 // Final pass over the slots set in `slots` (bit q): out[q] = sum, or max when
 // bit q of `maxes` is set, of the RED_BLOCKS partials of slot q.
