@@ -533,7 +533,9 @@ static void prepare_density_weights(amdk_ctx* c, const double* occ, int nb) {
 // the grid-sum partials in slot 8.
 static constexpr int RHO_GROUPS = 4;
 template <typename T>
-static void run_density(amdk_ctx* c, const void* C0, int nb, double* rho, bool norms_ready = false) {
+// norms_after: event after which slot 10 holds sum_b f_b <c_b|c_b> (side stream)
+static void run_density(amdk_ctx* c, const void* C0, int nb, double* rho, bool norms_ready = false,
+                        hipEvent_t norms_after = nullptr) {
   const Sphere& S = c->spw;
   const int npairs = (nb + 1) / 2;
   const size_t nr = (size_t)c->n1 * c->n2 * c->n3;
@@ -549,6 +551,7 @@ static void run_density(amdk_ctx* c, const void* C0, int nb, double* rho, bool n
     k_rho_combine<<<RED_BLOCKS, 256, 0, c->stream>>>(c->grid[1], nr, RHO_GROUPS, nr, rho, c->d_part + 8 * RED_BLOCKS);
     AMDK_TRY_HIP(hipGetLastError());
   }
+  if (norms_after) AMDK_TRY_HIP(hipStreamWaitEvent(c->stream, norms_after, 0));
   if (sizeof(T) == 4) renormalise_density<T>(c, C0, nb, rho, norms_ready, grouped);
 }
 
@@ -626,6 +629,9 @@ int amdk_ctx_create(const amdk_desc* d, amdk_ctx** out) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw Error(AMDK_ERR_DEVICE, "no HIP device available");
     AMDK_TRY_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    AMDK_TRY_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+    for (hipEvent_t* e : {&ctx->ev_fork, &ctx->ev_c0, &ctx->ev_nl})
+      AMDK_TRY_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
     build_sphere(ctx, d->miller_pw, d->ngpw, ctx->spw);
     build_sphere(ctx, d->miller_rho, d->ngrho, ctx->srho);
     // twiddles
@@ -686,6 +692,12 @@ void amdk_ctx_destroy(amdk_ctx* ctx) {
   for (void* p : ctx->owned) (void)hipFree(p);
   for (auto& s : ctx->pending) { (void)hipEventDestroy(s.a); (void)hipEventDestroy(s.b); }
   for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : {ctx->ev_fork, ctx->ev_c0, ctx->ev_nl})
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->side) {
+    (void)hipStreamSynchronize(ctx->side);
+    (void)hipStreamDestroy(ctx->side);
+  }
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
