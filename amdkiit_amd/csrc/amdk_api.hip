@@ -519,6 +519,7 @@ static void renormalise_density(amdk_ctx* c, const void* C0, int nb, double* rho
 // Band weights of the density on the device: d_wpair[b] = f_b / Omega (pairs
 // read two consecutive entries), d_wdens[b] = f_b.  Set once per occupation.
 static void prepare_density_weights(amdk_ctx* c, const double* occ, int nb) {
+  c->occ_state_loaded = false;
   std::vector<double> w(nb + 2, 0.0);
   for (int b = 0; b < nb; ++b) w[b] = occ[b] / c->omega;
   AMDK_TRY_HIP(hipMemcpyAsync(c->d_wpair, w.data(), sizeof(double) * (nb + 1), hipMemcpyHostToDevice, c->stream));

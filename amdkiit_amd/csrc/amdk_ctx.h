@@ -121,6 +121,7 @@ struct amdk_ctx {
   double* d_X = nullptr;      // nbmax x nbmax: U^-1 of the orthonormalisation
   double* d_nl = nullptr;     // P x nbmax
   double* d_Q = nullptr;      // max(P, nbmax) x nbmax
+  double* d_dfnl = nullptr;   // 3 x P x nbmax: <d beta / d R_k | C> per direction
   double* d_finv = nullptr;   // 1/f for PCG
   double* d_wnorm = nullptr;  // 1 for occupied bands (norm weights)
   double* d_wdens = nullptr;  // f_b for the density norm
@@ -135,6 +136,9 @@ struct amdk_ctx {
   void* Ct = nullptr;
   int nb_state = 0;
   std::vector<double> h_occ_state;
+  // d_occ, d_wnorm, d_finv and d_wpair hold the resident state's occupations
+  // (API calls with their own occupations overwrite them)
+  bool occ_state_loaded = false;
   // allocations to free
   std::vector<void*> owned;
   std::string err;

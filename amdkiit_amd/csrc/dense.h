@@ -378,19 +378,21 @@ __global__ void __launch_bounds__(256) k_c0_sums(const V* __restrict__ C, int ld
   }
 }
 
-// Local/Hartree force POTFOR (nuclear_grad.f90:4-50), one block per atom:
+// Local/Hartree force POTFOR (nuclear_grad.f90:4-50):
 // F_a = 2 Omega sum_{G != 0} Re[i G e^{-iGR_a} (rho_s 4pi/G^2 conj(rho_tot) + V_s conj(rho_e))]
+// Block (a, y) sums the G of split y (interleaved 256-chunks) into
+// part[(a * gridDim.y + y) * 3 + k] (without the 2 Omega).
 __global__ void __launch_bounds__(256) k_potfor(const int* __restrict__ mill, int ng, Recip rc,
                                                 const int* __restrict__ sp, const double* __restrict__ vps,
                                                 const double* __restrict__ rhos, int ldg,
                                                 const double2* __restrict__ rhog, const double2* __restrict__ sr,
-                                                const double2* __restrict__ tab, int W, int hmax, double omega,
-                                                double* __restrict__ force) {
+                                                const double2* __restrict__ tab, int W, int hmax,
+                                                double* __restrict__ part) {
   __shared__ double sh[4];
   const int a = blockIdx.x;
   const int s = sp[a];
   double f0 = 0, f1 = 0, f2 = 0;
-  for (int g = threadIdx.x + 1; g < ng; g += blockDim.x) {
+  for (int g = 1 + blockIdx.y * blockDim.x + threadIdx.x; g < ng; g += gridDim.y * blockDim.x) {
     double gx, gy, gz;
     gvec_of(mill, g, rc, gx, gy, gz);
     const double g2 = gx * gx + gy * gy + gz * gz;
@@ -410,9 +412,36 @@ __global__ void __launch_bounds__(256) k_potfor(const int* __restrict__ mill, in
   const double r1 = block_sum(f1, sh);
   const double r2 = block_sum(f2, sh);
   if (threadIdx.x == 0) {
-    force[3 * a] = 2.0 * omega * r0;
-    force[3 * a + 1] = 2.0 * omega * r1;
-    force[3 * a + 2] = 2.0 * omega * r2;
+    double* o = part + ((size_t)a * gridDim.y + blockIdx.y) * 3;
+    o[0] = r0;
+    o[1] = r1;
+    o[2] = r2;
+  }
+}
+
+// F_a,k = scale * sum_y part[a][y][k] (POTFOR, ordered) - 2 sum_{p of a} sum_b
+// f_b dfnl_k(p, b) Dnl(p, b) (NLFOR, nuclear_grad.f90:51-89; UPF_NLFOR with
+// D = hlsg).  dfnl_k and Dnl are column-major (P, nb), dfnl_k at dfnl + k P nb.
+__global__ void __launch_bounds__(256) k_force_final(const double* __restrict__ part, int ns, double scale, int P, int nb,
+                                                     const int* __restrict__ pstart, const double* __restrict__ occ,
+                                                     const double* __restrict__ dfnl, const double* __restrict__ dnl,
+                                                     double* __restrict__ force) {
+  __shared__ double sh[4];
+  const int a = blockIdx.x;
+  const int p0 = P > 0 ? pstart[a] : 0, np = P > 0 ? pstart[a + 1] - p0 : 0;
+  for (int k = 0; k < 3; ++k) {
+    double s = 0;
+    for (int e = threadIdx.x; e < np * nb; e += blockDim.x) {
+      const int p = p0 + e % np, b = e / np;
+      const size_t o = (size_t)b * P + p;
+      s += occ[b] * dfnl[(size_t)k * P * nb + o] * dnl[o];
+    }
+    const double nlf = block_sum(s, sh);
+    if (threadIdx.x == 0) {
+      double loc = 0;
+      for (int y = 0; y < ns; ++y) loc += part[((size_t)a * ns + y) * 3 + k];
+      force[3 * a + k] = scale * loc - 2.0 * nlf;
+    }
   }
 }
 

@@ -291,3 +291,27 @@ def test_mixed_matches_fp64_at_64_waters_converged():
     print(f"  |dE|/atom = {de:.2e}  max|dF| = {df:.2e}", flush=True)
     assert de < 1e-6
     assert df < 1e-4
+
+
+def test_stateless_calls_do_not_disturb_the_resident_state():
+    """amdk_lagrange / amdk_forces with their own occupations between resident
+    SCF steps: the steps continue exactly as without them."""
+    s = SMALL["h2o2_pz"]()
+    occ_other = s.occ.copy()
+    occ_other[-1] = 0.0
+    runs = []
+    for interleave in (False, True):
+        ctx = KSContext(s.pw, s.nbands, "fp64")
+        ctx.set_species(s.tables)
+        ctx.set_positions(s.sp, s.tau)
+        c = ctx.ortho(random_bands(s.pw, s.nbands, seed=30))
+        ctx.wfn_load(c, s.occ)
+        es = []
+        for it in range(1, 5):
+            es.append(ctx.scf_step(it, s.xc)["etot"])
+            if interleave:
+                ctx.lagrange(c, random_bands(s.pw, s.nbands, seed=31), occ_other)
+        runs.append((es, ctx.wfn_store()))
+        ctx.close()
+    assert runs[0][0] == runs[1][0]
+    assert np.array_equal(runs[0][1], runs[1][1])
