@@ -117,7 +117,7 @@ struct amdk_ctx {
   double* d_gpart = nullptr;  // split-K partials
   size_t gpart_cap = 0;
   double* d_S = nullptr;      // nbmax x nbmax
-  double* d_Ut = nullptr;     // nbmax x nbmax: Cholesky scratch (factor rows, block products)
+  double* d_Ut = nullptr;     // nbmax x nbmax: scratch of the blocked Cholesky inverse (block products)
   double* d_X = nullptr;      // nbmax x nbmax: U^-1 of the orthonormalisation
   double* d_nl = nullptr;     // P x nbmax
   double* d_Q = nullptr;      // max(P, nbmax) x nbmax
