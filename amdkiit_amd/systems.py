@@ -90,13 +90,16 @@ class SplitMix64:
         return np.array([self.next() for _ in range(n)])
 
 
-def water_box(lattice, box_bohr, ecut, xc="PBE", seed=2024, jitter=0.3):
+def water_box(lattice, box_bohr, ecut, xc="PBE", seed=2024, jitter=0.3, cell=None):
     """Seeded water box: O on an m1 x m2 x m3 lattice with U(-jitter, jitter)
     displacement, rigid monomer (O-H 0.9572 A, HOH 104.52 deg) with a uniform
-    random orientation (random unit quaternion)."""
+    random orientation (random unit quaternion).  `cell` (lattice vectors as
+    columns, bohr) replaces the cubic box; the O lattice is then laid out in
+    fractional coordinates."""
     rng = SplitMix64(seed)
     m1, m2, m3 = lattice
-    L = box_bohr
+    A = np.eye(3) * box_bohr if cell is None else np.asarray(cell, np.float64)
+    L = 1.0
     roh = 0.9572 * ANG2BOHR
     th = math.radians(104.52) / 2
     h1 = np.array([roh * math.sin(th), 0.0, roh * math.cos(th)])
@@ -105,7 +108,7 @@ def water_box(lattice, box_bohr, ecut, xc="PBE", seed=2024, jitter=0.3):
     for i in range(m1):
         for j in range(m2):
             for k in range(m3):
-                o = (np.array([i / m1, j / m2, k / m3]) + 0.5 / np.array([m1, m2, m3])) * L
+                o = A @ ((np.array([i / m1, j / m2, k / m3]) + 0.5 / np.array([m1, m2, m3])) * L)
                 o = o + (rng.uniform(3) * 2.0 - 1.0) * jitter
                 u1, u2, u3 = rng.uniform(3)
                 q = np.array([math.sqrt(1 - u1) * math.sin(2 * math.pi * u2), math.sqrt(1 - u1) * math.cos(2 * math.pi * u2),
@@ -118,7 +121,7 @@ def water_box(lattice, box_bohr, ecut, xc="PBE", seed=2024, jitter=0.3):
                 H.append(o + R @ h1)
                 H.append(o + R @ h2)
     nmol = m1 * m2 * m3
-    pw = PWSetup(Cell.cubic(L), ecut)
+    pw = PWSetup(Cell(A), ecut)
     species = [psp.gth("O", xc), psp.gth("H", xc)]
     tau = np.array(O + H)
     sp = [0] * nmol + [1] * (2 * nmol)

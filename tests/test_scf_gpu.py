@@ -34,6 +34,9 @@ SMALL = {
     "h2o2_pz": lambda: systems.water_box((1, 1, 2), 10.0, 20.0, "PZ"),
     "si8_lda": lambda: systems.si64_like(cells=1, ecut=12.0),
     "h2_upf": lambda: systems.h2_upf_small(),
+    # monoclinic-like skewed cell: non-orthogonal lattice, mixed FFT lengths
+    "h2o2_skew": lambda: systems.water_box((1, 1, 2), None, 20.0, "PBE",
+                                           cell=[[10.0, 1.2, 0.0], [0.0, 10.5, 0.9], [0.0, 0.0, 11.0]]),
 }
 TOL = {"fp64": 1e-11, "mixed": 3e-5}
 
