@@ -95,6 +95,7 @@ struct amdk_ctx {
   void* d_dbeta = nullptr;
   int4* d_plist = nullptr;
   double* d_vpp = nullptr;    // PCG preconditioner K(G)^-1 [ldc_pw]
+  double* d_g2pw = nullptr;   // |G|^2 on the wavefunction sphere [ldc_pw], 0 on the padding
   double* d_dmat = nullptr;   // [nsp][maxproj][maxproj]
   int* d_nproj = nullptr;     // [nsp]
   int* d_pstart = nullptr;    // [nat+1] first projector row of each atom

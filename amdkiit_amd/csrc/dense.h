@@ -353,20 +353,22 @@ __global__ void __launch_bounds__(256) k_grid_sum(const double* __restrict__ x, 
 // part10 = sum_b f_b DOTP(c_b, c_b).
 template <typename V>
 __global__ void __launch_bounds__(256) k_c0_sums(const V* __restrict__ C, int ldc, int npw, int nb,
-                                                 const int* __restrict__ mill, Recip rc,
-                                                 const double* __restrict__ occ, double* __restrict__ part4,
-                                                 double* __restrict__ part10) {
+                                                 const double* __restrict__ g2pw, const double* __restrict__ occ,
+                                                 double* __restrict__ part4, double* __restrict__ part10) {
+  // work item e = (band group, G): G fastest, so a wave reads consecutive
+  // coefficients of one band; C0_BG band groups keep every thread busy
+  constexpr int C0_BG = 8;
   __shared__ double sh[4];
   double s = 0, q = 0;
-  const int nch = (npw + 255) / 256, tot = nb * nch;
-  for (int ch = blockIdx.x; ch < tot; ch += gridDim.x) {
-    const int b = ch / nch, g = (ch - b * nch) * 256 + threadIdx.x;
-    if (g < npw) {
-      double gx, gy, gz;
-      gvec_of(mill, g, rc, gx, gy, gz);
+  const int nbg = (nb + C0_BG - 1) / C0_BG;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < (size_t)npw * C0_BG;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int g = (int)(e % npw), b0 = (int)(e / npw) * nbg, b1 = min(nb, b0 + nbg);
+    const double g2 = g2pw[g];
+    for (int b = b0; b < b1; ++b) {
       const V c = C[(size_t)b * ldc + g];
       const double a2 = (double)c.x * c.x + (double)c.y * c.y;
-      s += occ[b] * (gx * gx + gy * gy + gz * gz) * a2;
+      s += occ[b] * g2 * a2;
       q += occ[b] * (g == 0 ? (double)c.x * c.x : 2.0 * a2);
     }
   }
