@@ -318,3 +318,32 @@ def test_stateless_calls_do_not_disturb_the_resident_state():
         ctx.close()
     assert runs[0][0] == runs[1][0]
     assert np.array_equal(runs[0][1], runs[1][1])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_scf_step_with_chunked_pairs_matches_single_chunk(prec, monkeypatch):
+    """32 H2O (64 band pairs) with a work-array budget of a few pairs per
+    chunk: the density pass accumulates its pair groups over chunks and
+    H|psi> recomputes the planes per chunk; two SCF steps must match the
+    single-chunk run."""
+    s = systems.CONFIGS["h2o_32_pbe"]()
+    c0 = None
+    out = {}
+    for budget in (None, "200"):
+        if budget:
+            monkeypatch.setenv("AMDK_PAIR_BUDGET_MB", budget)
+        ctx = KSContext(s.pw, s.nbands, prec)
+        monkeypatch.delenv("AMDK_PAIR_BUDGET_MB", raising=False)
+        ctx.set_species(s.tables)
+        ctx.set_positions(s.sp, s.tau)
+        if c0 is None:
+            c0 = ctx.ortho(random_bands(s.pw, s.nbands, seed=33))
+        ctx.wfn_load(c0, s.occ)
+        es = [ctx.scf_step(it, s.xc) for it in (1, 2)]
+        print(f"  budget={budget} etot={es[-1]['etot']:.12f} nelec={es[-1]['nelec']:.12f}", flush=True)
+        out[budget] = (es, ctx.wfn_store())
+        ctx.close()
+    for k in ("etot", "ekin", "eh", "exc", "enl", "nelec"):
+        a, b = out[None][0][-1][k], out["200"][0][-1][k]
+        assert abs(a - b) < 1e-10 * max(1.0, abs(a)), (k, a, b)
+    assert rel(out["200"][1], out[None][1]) < 1e-9
