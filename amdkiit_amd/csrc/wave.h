@@ -34,21 +34,21 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
 
 constexpr int RED_BLOCKS = 1024;
 
-// Final pass over the slots set in `slots` (bit q): out[q] = sum, or max when
-// bit q of `maxes` is set, of the RED_BLOCKS partials of slot q.
+// Final pass over the slots set in `slots` (bit q), block q for slot q:
+// out[q] = sum, or max when bit q of `maxes` is set, of the RED_BLOCKS
+// partials of slot q.  Launched with 16 blocks.
 __global__ void k_red_slots(const double* __restrict__ part, unsigned slots, unsigned maxes, double* __restrict__ out) {
   __shared__ double sh[4];
-  for (int q = 0; q < 16; ++q) {
-    if (!((slots >> q) & 1u)) continue;
-    const bool is_max = (maxes >> q) & 1u;
-    double v = is_max ? -1.0 : 0.0;
-    for (int i = threadIdx.x; i < RED_BLOCKS; i += blockDim.x) {
-      const double x = part[(size_t)q * RED_BLOCKS + i];
-      v = is_max ? fmax(v, x) : v + x;
-    }
-    const double r = is_max ? block_max(v, sh) : block_sum(v, sh);
-    if (threadIdx.x == 0) out[q] = r;
+  const int q = blockIdx.x;
+  if (!((slots >> q) & 1u)) return;  // uniform over the block
+  const bool is_max = (maxes >> q) & 1u;
+  double v = is_max ? -1.0 : 0.0;
+  for (int i = threadIdx.x; i < RED_BLOCKS; i += blockDim.x) {
+    const double x = part[(size_t)q * RED_BLOCKS + i];
+    v = is_max ? fmax(v, x) : v + x;
   }
+  const double r = is_max ? block_max(v, sh) : block_sum(v, sh);
+  if (threadIdx.x == 0) out[q] = r;
 }
 
 // Band reductions walk (band, 256-wide chunk of G) pairs block by block, so
