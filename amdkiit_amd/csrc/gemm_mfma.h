@@ -61,11 +61,15 @@ __device__ __forceinline__ int xcd_block(int p, int per) { return (p & 7) * per 
 // Block tile 64 x 64 of the result, 4 waves of 32 x 32 (2 x 2 MFMA tiles);
 // logical block = (split, n-tile, m-tile), m fastest.  The next 64-deep K
 // tile is loaded into registers (double2, 512 B per row) while the current
-// one is multiplied from LDS.
+// one is multiplied from LDS.  sym (X = Y, M = N) computes only the tiles
+// on and above the diagonal (m-tile <= n-tile, numbered column by column);
+// k_ovl_reduce mirrors them.  A diagonal tile's two operands are the same
+// rows, loaded and staged once.
 template <typename TC>
 __global__ void __launch_bounds__(256) k_ovl_mfma(const double* __restrict__ X, int ldx, int M,
                                                   const double* __restrict__ Y, int ldy, int N, int K,
-                                                  int kchunk, int nsplit, int per, double* __restrict__ part) {
+                                                  int kchunk, int nsplit, int per, int sym,
+                                                  double* __restrict__ part) {
   typedef typename Acc4<TC>::type AccT;
   constexpr bool F64 = sizeof(TC) == 8;
   constexpr int KH = OVM_KT / 2;                      // double2 per row and tile
@@ -73,11 +77,23 @@ __global__ void __launch_bounds__(256) k_ovl_mfma(const double* __restrict__ X, 
   __shared__ TC xs[OVM_T][OVM_KT + 1];
   __shared__ TC ys[OVM_T][OVM_KT + 1];
   const int tm = (M + OVM_T - 1) / OVM_T, tn = (N + OVM_T - 1) / OVM_T;
+  const int tiles = sym ? tn * (tn + 1) / 2 : tm * tn;
   const int L = xcd_block(blockIdx.x, per);
-  if (L >= tm * tn * nsplit) return;
-  const int split = L / (tm * tn), t = L % (tm * tn);
-  const int m0 = (t % tm) * OVM_T, n0 = (t / tm) * OVM_T;
+  if (L >= tiles * nsplit) return;
+  const int split = L / tiles, t = L % tiles;
+  int m0, n0;
+  if (sym) {  // t = j (j + 1) / 2 + i, i <= j
+    int j = 0;
+    while ((j + 1) * (j + 2) / 2 <= t) ++j;
+    m0 = (t - j * (j + 1) / 2) * OVM_T;
+    n0 = j * OVM_T;
+  } else {
+    m0 = (t % tm) * OVM_T;
+    n0 = (t / tm) * OVM_T;
+  }
   const int kb = split * kchunk, ke = min(K, kb + kchunk);
+  const bool diag = sym && m0 == n0;  // uniform per block
+  const TC(*yt)[OVM_KT + 1] = diag ? xs : ys;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
   const int li = lane & 15, lk = lane >> 4;
@@ -97,7 +113,8 @@ __global__ void __launch_bounds__(256) k_ovl_mfma(const double* __restrict__ X, 
       const int e = threadIdx.x + 256 * it, r = e / KH, k = k0 + 2 * (e % KH);
       // K and every chunk start are even, so a pair never straddles ke
       px[it] = (m0 + r < M && k < ke) ? *(const double2*)(X + (size_t)(m0 + r) * ldx + k) : make_double2(0.0, 0.0);
-      py[it] = (n0 + r < N && k < ke) ? *(const double2*)(Y + (size_t)(n0 + r) * ldy + k) : make_double2(0.0, 0.0);
+      if (!diag)
+        py[it] = (n0 + r < N && k < ke) ? *(const double2*)(Y + (size_t)(n0 + r) * ldy + k) : make_double2(0.0, 0.0);
     }
   };
   load(kb);
@@ -107,8 +124,10 @@ __global__ void __launch_bounds__(256) k_ovl_mfma(const double* __restrict__ X, 
       const int e = threadIdx.x + 256 * it, r = e / KH, c = 2 * (e % KH);
       xs[r][c] = (TC)px[it].x;
       xs[r][c + 1] = (TC)px[it].y;
-      ys[r][c] = (TC)py[it].x;
-      ys[r][c + 1] = (TC)py[it].y;
+      if (!diag) {
+        ys[r][c] = (TC)py[it].x;
+        ys[r][c + 1] = (TC)py[it].y;
+      }
     }
     __syncthreads();
     if (k0 + OVM_KT < ke) load(k0 + OVM_KT);
@@ -116,7 +135,7 @@ __global__ void __launch_bounds__(256) k_ovl_mfma(const double* __restrict__ X, 
     for (int ks = 0; ks < OVM_KT; ks += 4) {
       const int kk = ks + lk;
       const TC a0 = xs[wm + li][kk], a1 = xs[wm + 16 + li][kk];
-      const TC b0 = ys[wn + li][kk], b1 = ys[wn + 16 + li][kk];
+      const TC b0 = yt[wn + li][kk], b1 = yt[wn + 16 + li][kk];
       if constexpr (F64) {
         acc[0][0] = mfma16(a0, b0, acc[0][0]);
         acc[0][1] = mfma16(a0, b1, acc[0][1]);
@@ -163,14 +182,16 @@ constexpr int UPM_MC = 32, UPM_NB = 128, UPM_K = 64;
 // over the stacked rows (part 1 padded with zero rows to a whole 32-row
 // chunk), with Q1 scaled by a1 and Q2 by a2, so that two updates of Y share
 // one pass over it.  zero_g0_im stores 0 into real row 1 of every column
-// (Im c(G = 0) of a Gamma-point band, which must vanish).
+// (Im c(G = 0) of a Gamma-point band, which must vanish).  tri declares Q1
+// upper triangular (Q1(m, n) = 0 for m > n): the MFMA tiles whose columns
+// all lie left of a chunk's first row are skipped (their terms are zero).
 template <typename TC>
 __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X1, int ldx, int M1,
                                                   const double* __restrict__ Q1, int ldq1, double a1,
                                                   const double* __restrict__ X2, int M2,
                                                   const double* __restrict__ Q2, int ldq2, double a2,
                                                   double* __restrict__ Y, int ldy, int N, int K,
-                                                  double beta, int per, int zero_g0_im) {
+                                                  double beta, int per, int zero_g0_im, int tri) {
   typedef typename Acc4<TC>::type AccT;
   constexpr bool F64 = sizeof(TC) == 8;
   constexpr int XL = (UPM_MC * UPM_K / 2) / 256;   // double2 of X per thread
@@ -233,14 +254,18 @@ __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X1,
     __syncthreads();
     if (mb + UPM_MC < M) load(mb + UPM_MC);
     const int mend = min(UPM_MC, M - mb);
+    // tiles t < tskip: columns n0 + 16 t + 15 < mb, Q1 = 0 on this chunk
+    const int tskip = (tri && mb < M1p) ? min(8, max(0, (mb - n0) / 16)) : 0;
     for (int ms = 0; ms < mend; ms += 4) {
       const TC b = xs[ms + lk][kl];
       if constexpr (F64) {
 #pragma unroll
-        for (int t = 0; t < 8; ++t) acc[t] = mfma16(qs[ms + lk][16 * t + li], b, acc[t]);
+        for (int t = 0; t < 8; ++t)
+          if (t >= tskip) acc[t] = mfma16(qs[ms + lk][16 * t + li], b, acc[t]);
       } else {
 #pragma unroll
-        for (int t = 0; t < 8; ++t) tac[t] = mfma16(qs[ms + lk][16 * t + li], b, tac[t]);
+        for (int t = 0; t < 8; ++t)
+          if (t >= tskip) tac[t] = mfma16(qs[ms + lk][16 * t + li], b, tac[t]);
       }
     }
     if constexpr (!F64) {

@@ -159,6 +159,18 @@ def gemm_table(prof, pw, nb, nproj, prec, nsteps):
         "ortho_ovl": (nb, nb, 2 * pw.ngpw, "f64"),
         "ortho_trmm": (nb, nb, 2 * ldc, "f64"),
     }
+    # the orthonormalisation computes only the tiles the product needs: the
+    # 64 x 64 overlap tiles on and above the diagonal (S is symmetric) and,
+    # in C U^-1, the 32-row x 16-column MFMA tiles not below U^-1's diagonal
+    tn = (nb + 63) // 64
+    pairs = done = 0
+    for n0 in range(0, nb, 128):
+        for mb in range(0, (nb + 31) // 32 * 32, 32):
+            for t in range(8):
+                if n0 + 16 * t < nb:
+                    pairs += 1
+                    done += t >= min(8, max(0, int((mb - n0) / 16)))
+    executed = {"ortho_ovl": tn * (tn + 1) / 2 / tn ** 2, "ortho_trmm": done / pairs}
     out = {}
     for k, (M, N, K, ar) in shapes.items():
         if k not in prof or not prof[k][1] or M == 0:
@@ -167,6 +179,9 @@ def gemm_table(prof, pw, nb, nproj, prec, nsteps):
         tf = 2.0 * M * N * K / (ms * 1e-3) / 1e12
         out[k] = {"M": M, "N": N, "K": K, "arith": ar, "avg_ms": ms, "TFLOPs": tf,
                   "peak": MFMA_PEAK[ar], "frac": tf / MFMA_PEAK[ar]}
+        if k in executed:  # TFLOPs above counts the full product
+            out[k]["executed_fraction"] = executed[k]
+            out[k]["frac_executed"] = tf * executed[k] / MFMA_PEAK[ar]
     return out
 
 
