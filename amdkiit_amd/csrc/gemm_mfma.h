@@ -182,16 +182,16 @@ constexpr int UPM_MC = 32, UPM_NB = 128, UPM_K = 64;
 // over the stacked rows (part 1 padded with zero rows to a whole 32-row
 // chunk), with Q1 scaled by a1 and Q2 by a2, so that two updates of Y share
 // one pass over it.  zero_g0_im stores 0 into real row 1 of every column
-// (Im c(G = 0) of a Gamma-point band, which must vanish).  tri declares Q1
+// (Im c(G = 0) of a Gamma-point band, which must vanish).  TRI declares Q1
 // upper triangular (Q1(m, n) = 0 for m > n): the MFMA tiles whose columns
 // all lie left of a chunk's first row are skipped (their terms are zero).
-template <typename TC>
+template <typename TC, bool TRI>
 __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X1, int ldx, int M1,
                                                   const double* __restrict__ Q1, int ldq1, double a1,
                                                   const double* __restrict__ X2, int M2,
                                                   const double* __restrict__ Q2, int ldq2, double a2,
                                                   double* __restrict__ Y, int ldy, int N, int K,
-                                                  double beta, int per, int zero_g0_im, int tri) {
+                                                  double beta, int per, int zero_g0_im) {
   typedef typename Acc4<TC>::type AccT;
   constexpr bool F64 = sizeof(TC) == 8;
   constexpr int XL = (UPM_MC * UPM_K / 2) / 256;   // double2 of X per thread
@@ -255,17 +255,17 @@ __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X1,
     if (mb + UPM_MC < M) load(mb + UPM_MC);
     const int mend = min(UPM_MC, M - mb);
     // tiles t < tskip: columns n0 + 16 t + 15 < mb, Q1 = 0 on this chunk
-    const int tskip = (tri && mb < M1p) ? min(8, max(0, (mb - n0) / 16)) : 0;
+    const int tskip = (TRI && mb < M1p) ? min(8, max(0, (mb - n0) / 16)) : 0;
     for (int ms = 0; ms < mend; ms += 4) {
       const TC b = xs[ms + lk][kl];
       if constexpr (F64) {
 #pragma unroll
         for (int t = 0; t < 8; ++t)
-          if (t >= tskip) acc[t] = mfma16(qs[ms + lk][16 * t + li], b, acc[t]);
+          if (!TRI || t >= tskip) acc[t] = mfma16(qs[ms + lk][16 * t + li], b, acc[t]);
       } else {
 #pragma unroll
         for (int t = 0; t < 8; ++t)
-          if (t >= tskip) tac[t] = mfma16(qs[ms + lk][16 * t + li], b, tac[t]);
+          if (!TRI || t >= tskip) tac[t] = mfma16(qs[ms + lk][16 * t + li], b, tac[t]);
       }
     }
     if constexpr (!F64) {
