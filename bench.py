@@ -8,7 +8,10 @@ energies).  One "step" is one iteration of the reference's wfn_opt loop
 device-resident wavefunctions.  Synthetic data: seeded water lattice and
 seeded random initial coefficients.
 
-Prints one JSON line (see DESIGN.md for the fields).
+Prints one JSON line: the steps/s figure with the step time, the H|psi>
+roofline (algorithmic bytes per launch over the HIP-event launch time,
+against 8 TB/s), the per-region and per-GEMM times, the FP64 comparison,
+the 64 H2O figures and the CPU baseline.
 """
 import argparse
 import json
@@ -292,8 +295,8 @@ def main():
         print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": 5}))
         return
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
-              "the multi-GPU band sharding is a design in DESIGN.md, not built.", file=sys.stderr)
+        print(f"bench.py: this implementation runs on one GPU only (asked for --gpus {a.gpus}); "
+              "splitting the bands over GPUs is designed but not built.", file=sys.stderr)
         sys.exit(2)
 
     from amdkiit_amd import systems
