@@ -32,6 +32,7 @@ class KSContext:
         d.miller_rho = L.iptr(self._mrho)
         d.nbmax = self.nbmax
         d.precision = self.precision
+        d.symmetry = getattr(pw.cell, "symmetry", 0)
         h = C.c_void_p()
         rc = self.lib.amdk_ctx_create(C.byref(d), C.byref(h))
         self.h = h

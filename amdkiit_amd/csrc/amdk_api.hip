@@ -242,32 +242,32 @@ __global__ void k_scale_copy(const double* __restrict__ src, T* __restrict__ dst
     dst[i] = (T)(src[i] * s);
 }
 
+// Caller-layout FP64 host arrays <-> internal device layout.  The caller's
+// first npw coefficients of each band are copied with their pitch into the
+// context's staging array (pitch ldc), then permuted on the device; nothing
+// is allocated per call.  Entries of the caller's array beyond npw are
+// neither read nor written.
 static void bands_in(amdk_ctx* c, const double* host, int ldsrc, int nb, void* dst) {
   const Sphere& S = c->spw;
-  const size_t nsrc = (size_t)ldsrc * nb * 2;
-  double* stage = nullptr;
-  AMDK_TRY_HIP(hipMalloc(&stage, sizeof(double) * std::max<size_t>(nsrc, 2)));
-  AMDK_TRY_HIP(hipMemcpyAsync(stage, host, sizeof(double) * nsrc, hipMemcpyHostToDevice, c->stream));
+  AMDK_TRY_HIP(hipMemcpy2DAsync(c->d_stage, sizeof(double2) * S.ldc, host, sizeof(double2) * ldsrc,
+                                sizeof(double2) * S.npw, nb, hipMemcpyHostToDevice, c->stream));
   AMDK_TRY_HIP(hipMemsetAsync(dst, 0, sbytes(c) * (size_t)S.ldc * nb, c->stream));
   dim3 grid((S.npw + 255) / 256, nb);
-  k_from_caller<double2><<<grid, 256, 0, c->stream>>>((const double2*)stage, ldsrc, (double2*)dst, S.ldc, S.d_from_caller, S.d_conj_caller, S.npw, nb);
+  k_from_caller<double2><<<grid, 256, 0, c->stream>>>((const double2*)c->d_stage, S.ldc, (double2*)dst, S.ldc,
+                                                      S.d_from_caller, S.d_conj_caller, S.npw, nb);
   AMDK_TRY_HIP(hipGetLastError());
   AMDK_TRY_HIP(hipStreamSynchronize(c->stream));
-  AMDK_TRY_HIP(hipFree(stage));
 }
 
 static void bands_out(amdk_ctx* c, const void* src, int nb, double* host, int lddst) {
   const Sphere& S = c->spw;
-  const size_t ndst = (size_t)lddst * nb * 2;
-  double* stage = nullptr;
-  AMDK_TRY_HIP(hipMalloc(&stage, sizeof(double) * std::max<size_t>(ndst, 2)));
-  AMDK_TRY_HIP(hipMemsetAsync(stage, 0, sizeof(double) * ndst, c->stream));
   dim3 grid((S.npw + 255) / 256, nb);
-  k_to_caller<double2><<<grid, 256, 0, c->stream>>>((const double2*)src, S.ldc, (double2*)stage, lddst, S.d_from_caller, S.d_conj_caller, S.npw, nb);
+  k_to_caller<double2><<<grid, 256, 0, c->stream>>>((const double2*)src, S.ldc, (double2*)c->d_stage, S.ldc,
+                                                    S.d_from_caller, S.d_conj_caller, S.npw, nb);
   AMDK_TRY_HIP(hipGetLastError());
-  AMDK_TRY_HIP(hipMemcpyAsync(host, stage, sizeof(double) * ndst, hipMemcpyDeviceToHost, c->stream));
+  AMDK_TRY_HIP(hipMemcpy2DAsync(host, sizeof(double2) * lddst, c->d_stage, sizeof(double2) * S.ldc,
+                                sizeof(double2) * S.npw, nb, hipMemcpyDeviceToHost, c->stream));
   AMDK_TRY_HIP(hipStreamSynchronize(c->stream));
-  AMDK_TRY_HIP(hipFree(stage));
 }
 
 // ---------------------------------------------------------------------------
@@ -627,6 +627,27 @@ int amdk_ctx_create(const amdk_desc* d, amdk_ctx** out) {
     const double tpi = 2.0 * M_PI;
     for (int j = 0; j < 3; ++j)
       for (int i = 0; i < 3; ++i) ctx->rc.g[3 * j + i] = tpi * inv[j][i];
+    // minimum image of E_SR
+    {
+      const bool cubic = std::fabs(A(0, 1)) + std::fabs(A(0, 2)) + std::fabs(A(1, 0)) + std::fabs(A(1, 2)) +
+                                 std::fabs(A(2, 0)) + std::fabs(A(2, 1)) < 1e-12 &&
+                         std::fabs(A(0, 0) - A(1, 1)) < 1e-12 && std::fabs(A(0, 0) - A(2, 2)) < 1e-12;
+      if (d->symmetry < 0) throw Error(AMDK_ERR_ARG, "bad lattice symmetry code");
+      PbcInfo& pb = ctx->pbc;
+      for (int k = 0; k < 9; ++k) pb.a[k] = a[k];
+      for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) pb.ainv[3 * j + i] = inv[j][i];
+      if (d->symmetry == 2) {
+        pb.mode = 2;  // conventional constant from |a_1| = A / sqrt2
+        pb.A = std::sqrt(2.0 * (A(0, 0) * A(0, 0) + A(1, 0) * A(1, 0) + A(2, 0) * A(2, 0)));
+      } else if ((d->symmetry == 0 || d->symmetry == 1) && cubic) {
+        pb.mode = 1;
+        pb.A = A(0, 0);
+      } else {
+        pb.mode = 3;
+        pb.A = 0.0;
+      }
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw Error(AMDK_ERR_DEVICE, "no HIP device available");
     AMDK_TRY_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
@@ -721,8 +742,7 @@ int amdk_set_vks(amdk_ctx* ctx, const double* v_r) {
   if (!v_r) throw Error(AMDK_ERR_ARG, "null v_r");
   const size_t nr = (size_t)ctx->n1 * ctx->n2 * ctx->n3;
   check_finite(v_r, nr, "v_r");
-  double* stage = nullptr;
-  AMDK_TRY_HIP(hipMalloc(&stage, sizeof(double) * nr));
+  double* stage = ctx->grid[1];  // dense scratch grid, free between calls
   AMDK_TRY_HIP(hipMemcpyAsync(stage, v_r, sizeof(double) * nr, hipMemcpyHostToDevice, ctx->stream));
   const double s = 1.0 / (double)nr;  // forward FFT normalisation folded into V
   if (ctx->prec == AMDK_PREC_FP64)
@@ -731,7 +751,6 @@ int amdk_set_vks(amdk_ctx* ctx, const double* v_r) {
     k_scale_copy<float><<<1024, 256, 0, ctx->stream>>>(stage, (float*)ctx->vr, nr, s);
   AMDK_TRY_HIP(hipGetLastError());
   AMDK_TRY_HIP(hipStreamSynchronize(ctx->stream));
-  AMDK_TRY_HIP(hipFree(stage));
   ctx->have_vks = true;
   API_END
 }
@@ -743,6 +762,7 @@ int amdk_vpsi_dev(amdk_ctx* ctx, const void* c0, void* c2, const double* occ, in
   if (!c0 || !c2 || !occ) throw Error(AMDK_ERR_ARG, "null pointer");
   if (!ctx->have_vks) throw Error(AMDK_ERR_STATE, "V_KS not set (amdk_set_vks)");
   AMDK_TRY_HIP(hipMemcpyAsync(ctx->d_occ, occ, sizeof(double) * nb, hipMemcpyHostToDevice, ctx->stream));
+  ctx->occ_state_loaded = false;  // d_occ now holds this call's occupations
   if (ctx->prec == AMDK_PREC_FP64) run_vpsi<double>(ctx, c0, c2, nb);
   else run_vpsi<float>(ctx, c0, c2, nb);
   sync(ctx);
@@ -756,19 +776,16 @@ int amdk_vpsi(amdk_ctx* ctx, const double* c0, int ldc, double* c2, const double
   if (ldc < ctx->spw.npw) throw Error(AMDK_ERR_ARG, "ldc < ngpw");
   if (!ctx->have_vks) throw Error(AMDK_ERR_STATE, "V_KS not set (amdk_set_vks)");
   check_finite(c0, (size_t)2 * ldc * nb, "c0");
-  const size_t bytes = sbytes(ctx) * (size_t)ctx->spw.ldc * nb;
-  void *dc0 = nullptr, *dc2 = nullptr;
-  AMDK_TRY_HIP(hipMalloc(&dc0, bytes));
-  AMDK_TRY_HIP(hipMalloc(&dc2, bytes));
+  // band scratch of the context (C2 and Ct carry no state between calls)
+  void* dc0 = ctx->Ct;
+  void* dc2 = ctx->C2;
   bands_in(ctx, c0, ldc, nb, dc0);
-  AMDK_TRY_HIP(hipMemsetAsync(dc2, 0, bytes, ctx->stream));
   AMDK_TRY_HIP(hipMemcpyAsync(ctx->d_occ, occ, sizeof(double) * nb, hipMemcpyHostToDevice, ctx->stream));
+  ctx->occ_state_loaded = false;  // d_occ now holds this call's occupations
   if (ctx->prec == AMDK_PREC_FP64) run_vpsi<double>(ctx, dc0, dc2, nb);
   else run_vpsi<float>(ctx, dc0, dc2, nb);
   sync(ctx);
   bands_out(ctx, dc2, nb, c2, ldc);
-  AMDK_TRY_HIP(hipFree(dc0));
-  AMDK_TRY_HIP(hipFree(dc2));
   API_END
 }
 
@@ -790,20 +807,15 @@ int amdk_density(amdk_ctx* ctx, const double* c0, int ldc, const double* occ, in
   if (!c0 || !occ || !rho_r) throw Error(AMDK_ERR_ARG, "null pointer");
   if (ldc < ctx->spw.npw) throw Error(AMDK_ERR_ARG, "ldc < ngpw");
   check_finite(c0, (size_t)2 * ldc * nb, "c0");
-  const size_t bytes = sbytes(ctx) * (size_t)ctx->spw.ldc * nb;
   const size_t nr = (size_t)ctx->n1 * ctx->n2 * ctx->n3;
-  void* dc0 = nullptr;
-  double* drho = nullptr;
-  AMDK_TRY_HIP(hipMalloc(&dc0, bytes));
-  AMDK_TRY_HIP(hipMalloc(&drho, sizeof(double) * nr));
+  void* dc0 = ctx->Ct;          // band scratch
+  double* drho = ctx->grid[0];  // density grid (rewritten before every use)
   bands_in(ctx, c0, ldc, nb, dc0);
   prepare_density_weights(ctx, occ, nb);
   if (ctx->prec == AMDK_PREC_FP64) run_density<double>(ctx, dc0, nb, drho);
   else run_density<float>(ctx, dc0, nb, drho);
   sync(ctx);
   download(ctx, rho_r, drho, sizeof(double) * nr);
-  AMDK_TRY_HIP(hipFree(dc0));
-  AMDK_TRY_HIP(hipFree(drho));
   API_END
 }
 

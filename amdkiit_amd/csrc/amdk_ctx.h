@@ -45,6 +45,7 @@ struct amdk_ctx {
   double acell[9];
   double omega = 0;
   amdk::Recip rc;
+  amdk::PbcInfo pbc;          // minimum-image rule of E_SR (from amdk_desc.symmetry)
   int prec = AMDK_PREC_FP64;
   bool fft_generic = false;   // AMDK_FFT=stockham forces the LDS Stockham passes
   int nbmax = 0;
@@ -103,6 +104,9 @@ struct amdk_ctx {
   std::vector<int> h_pstart;
   double esr = 0, eself = 0;
   std::vector<double> h_fsr;
+  double* d_gm = nullptr;     // [9] reciprocal rows for the phase tables
+  double* d_fsr = nullptr;    // [3 nat] E_SR forces
+  double* d_esr = nullptr;    // [nat] E_SR per-atom partials
   // dense FP64 workspaces (density sphere)
   double2* W1r = nullptr;
   double2* W2r = nullptr;
@@ -112,6 +116,8 @@ struct amdk_ctx {
   double2* d_fld0 = nullptr;  // 2 x ldc_rho
   double2* d_fld1 = nullptr;
   bool have_rhog = false;
+  // d_rhog was formed by the last SCF step (not by a stateless amdk_vks)
+  bool rhog_from_state = false;
   // reductions and band-space matrices
   double* d_part = nullptr;   // [8][RED_BLOCKS]
   double* d_red = nullptr;    // [16]
@@ -135,6 +141,7 @@ struct amdk_ctx {
   void* Hd = nullptr;
   void* Pd = nullptr;
   void* Ct = nullptr;
+  void* d_stage = nullptr;  // caller-layout staging of host band arrays (ldc x nbmax)
   int nb_state = 0;
   std::vector<double> h_occ_state;
   // d_occ, d_wnorm, d_finv and d_wpair hold the resident state's occupations

@@ -447,11 +447,59 @@ __global__ void __launch_bounds__(256) k_force_final(const double* __restrict__ 
   }
 }
 
-// Short-range ion-ion energy and forces, minimum image for cubic cells
-// (e_sr, total_energy.f90:133-239).  One thread per atom i, ordered sum over j;
-// energy counted for j > i.
+// Minimum-image rule of the ion-ion term (apply_pbc.f90:1-60 with m = 1):
+// mode 1 simple cubic (each component folded into (-A/2, A/2]); mode 2 FCC
+// with conventional constant A: rotated by 45 degrees about z into the
+// tetragonal frame of the lattice, folded into its box and moved by the
+// body-centre vector when outside |x| + |y| + sqrt2 |z| < 2 m_a; mode 3 the
+// nearest image in fractional coordinates (s -= NINT(s)), as the reference
+// does for the other lattices.  a: lattice vectors as columns (a[3 j + i] =
+// (a_j)_i), ainv: rows of the inverse.
+struct PbcInfo {
+  int mode;
+  double A;
+  double a[9];
+  double ainv[9];
+};
+
+__device__ __forceinline__ void min_image(const PbcInfo& p, double d[3]) {
+  if (p.mode == 1) {
+    for (int k = 0; k < 3; ++k) {
+      const double x = fmod(d[k], p.A);
+      d[k] = x - p.A * trunc(x / (0.5 * p.A));
+    }
+  } else if (p.mode == 2) {
+    const double s2 = sqrt(2.0), ma = p.A * s2 / 4.0;
+    double x = (d[0] + d[1]) / s2, y = (-d[0] + d[1]) / s2, z = d[2];
+    x = fmod(x, 2.0 * ma);
+    y = fmod(y, 2.0 * ma);
+    z = fmod(z, 2.0 * ma * s2);
+    x -= 2.0 * ma * trunc(x / ma);
+    y -= 2.0 * ma * trunc(y / ma);
+    z -= 2.0 * ma * s2 * trunc(z / (s2 * ma));
+    if (!(fabs(x) + fabs(y) + s2 * fabs(z) < 2.0 * ma)) {
+      x -= copysign(ma, x);
+      y -= copysign(ma, y);
+      z -= copysign(s2 * ma, z);
+    }
+    d[0] = (x - y) / s2;
+    d[1] = (x + y) / s2;
+    d[2] = z;
+  } else if (p.mode == 3) {
+    double s[3];
+    for (int j = 0; j < 3; ++j) {
+      const double v = p.ainv[3 * j] * d[0] + p.ainv[3 * j + 1] * d[1] + p.ainv[3 * j + 2] * d[2];
+      s[j] = v - copysign(floor(fabs(v) + 0.5), v);  // NINT: halves away from zero
+    }
+    for (int i = 0; i < 3; ++i) d[i] = p.a[i] * s[0] + p.a[3 + i] * s[1] + p.a[6 + i] * s[2];
+  }
+}
+
+// Short-range ion-ion energy and forces with the minimum image above
+// (e_sr, total_energy.f90:133-239, nlp = 0).  One thread per atom i,
+// ordered sum over j; energy counted for j > i.
 __global__ void k_esr(const double* __restrict__ tau, int nat, const int* __restrict__ sp,
-                      const double* __restrict__ zv, double L, int cubic, double rckj,
+                      const double* __restrict__ zv, PbcInfo pbc, double rckj,
                       double* __restrict__ force, double* __restrict__ epart) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nat) return;
@@ -459,12 +507,7 @@ __global__ void k_esr(const double* __restrict__ tau, int nat, const int* __rest
   for (int j = 0; j < nat; ++j) {
     if (j == i) continue;
     double d[3] = {tau[3 * i] - tau[3 * j], tau[3 * i + 1] - tau[3 * j + 1], tau[3 * i + 2] - tau[3 * j + 2]};
-    if (cubic) {
-      for (int k = 0; k < 3; ++k) {
-        double x = fmod(d[k], L);
-        d[k] = x - L * trunc(x / (0.5 * L));
-      }
-    }
+    min_image(pbc, d);
     const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
     const double r = sqrt(r2);
     const double arg = r / rckj;

@@ -318,7 +318,9 @@ __device__ __forceinline__ double bcast_lane(double v, int l) {
 // T -= F R (upper) or T -= F RM (lower), F(i, q) = R(q, i) / d_q, and tiles
 // that it leaves unchanged are skipped.  A non-positive pivot k sets
 // info = info_base + k + 1 and the whole block returns; the block with
-// info_base 0 first clears info.
+// info_base 0 first clears info, and a later block (info_base > 0) does
+// nothing when an earlier one has already failed, so info names the first
+// failing leading minor.
 template <int NT, int PB>
 __global__ void __launch_bounds__(64 * NT) k_chol_inv(const double* __restrict__ A, int lda, int n,
                                                      double* __restrict__ X, int ldx, int nz,
@@ -334,6 +336,8 @@ __global__ void __launch_bounds__(64 * NT) k_chol_inv(const double* __restrict__
   const int tid = threadIdx.x, lane = tid & 63, J = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int li = lane & 15, lk = lane >> 4, xj = 16 * J + li;  // this lane's column in its tiles
   const bool jlive = 16 * J < n;
+  // earlier leaves ran before this launch on the same stream: the same value in every thread
+  if (info_base > 0 && *(volatile const int*)info != 0) return;
   if (tid == 0) {
     fail = 0;
     if (info_base == 0) *info = 0;

@@ -44,8 +44,7 @@ class GPUForceProvider:
             e, fi = self.ctx.scf_step_tfor(it, s.xc, want)
             if self.outputs is not None:
                 self.outputs.sp_energy(it, e["etot"], 1e3 * (time.perf_counter() - t0))
-            if fi is not None:
-                f = fi
+            f = fi  # forces only of this step's state, never an earlier one
             prev = e["gemax"]
             if e["gemax"] < self.gcut and f is not None:
                 self.nscf.append(it)

@@ -22,7 +22,7 @@ module amdk_hip
     type(c_ptr) :: miller_pw
     integer(c_int) :: ngrho
     type(c_ptr) :: miller_rho
-    integer(c_int) :: nbmax, precision
+    integer(c_int) :: nbmax, precision, symmetry
   end type
 
   type, bind(C), public :: amdk_species

@@ -51,7 +51,7 @@ program amdk_smoke
   d%a_cell(1) = alat; d%a_cell(5) = alat; d%a_cell(9) = alat
   d%ngpw = ng; d%miller_pw = c_loc(mill)
   d%ngrho = ngr; d%miller_rho = c_loc(millr)
-  d%nbmax = nb; d%precision = AMDK_PREC_FP64
+  d%nbmax = nb; d%precision = AMDK_PREC_FP64; d%symmetry = 1
   rc = amdk_ctx_create(d, ctx)
   call amdk_check(ctx, rc, 'amdk_ctx_create')
 

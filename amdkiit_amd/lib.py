@@ -29,7 +29,7 @@ class Desc(C.Structure):
         ("a_cell", C.c_double * 9),
         ("ngpw", C.c_int), ("miller_pw", _ip),
         ("ngrho", C.c_int), ("miller_rho", _ip),
-        ("nbmax", C.c_int), ("precision", C.c_int),
+        ("nbmax", C.c_int), ("precision", C.c_int), ("symmetry", C.c_int),
     ]
 
 

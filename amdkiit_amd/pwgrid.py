@@ -40,17 +40,42 @@ def grid_size(length_bohr, ecut_rho_ry):
 
 
 class Cell:
-    """Lattice vectors as columns of `a` (bohr)."""
+    """Lattice vectors as columns of `a` (bohr).  `symmetry` is the
+    reference's lattice code (set_cell.f90:17-42: 1 cubic, 2 FCC, 3 BCC, ...),
+    which selects the minimum-image rule of the ion-ion term
+    (apply_pbc.f90); 0 means "not given": cubic when the cell is, otherwise
+    the nearest image in fractional coordinates."""
 
-    def __init__(self, a):
+    def __init__(self, a, symmetry=0):
         self.a = np.array(a, dtype=np.float64).reshape(3, 3)
         self.omega = abs(np.linalg.det(self.a))
         # g_j = 2 pi * row j of inv(A)
         self.g = 2.0 * np.pi * np.linalg.inv(self.a)  # rows are g1, g2, g3
+        self.symmetry = int(symmetry)
 
     @classmethod
     def cubic(cls, length):
-        return cls(np.eye(3) * length)
+        return cls(np.eye(3) * length, symmetry=1)
+
+    @classmethod
+    def fcc(cls, a):
+        """Face-centred cubic with conventional constant a (bohr): the
+        primitive vectors of set_vec (set_cell.f90:49-52) as columns."""
+        h = a / 2.0
+        return cls(np.array([[-h, 0.0, h], [0.0, h, h], [-h, h, 0.0]]).T, symmetry=2)
+
+    @property
+    def is_cubic(self):
+        L = np.diag(self.a)
+        return bool(np.allclose(self.a, np.diag(L), rtol=0, atol=1e-12) and np.allclose(L, L[0], rtol=0, atol=1e-12))
+
+    def pbc_mode(self):
+        """Minimum-image rule: 1 cubic box, 2 FCC Wigner-Seitz, 3 fractional."""
+        if self.symmetry == 2:
+            return 2
+        if self.symmetry in (0, 1) and self.is_cubic:
+            return 1
+        return 3
 
     def gvec(self, mill):
         """Cartesian G (bohr^-1) for Miller indices (N,3)."""

@@ -52,6 +52,19 @@ def reference_water(nmol, ecut=80.0, box_ang=9.85):
     return System(pw, [sO, sH], sp, tau, nb, np.full(nb, 2.0), "TETER93", f"h2o_{nmol}_ref")
 
 
+def reference_si2():
+    """Recorded FCC silicon runs (tests/bulk_si/{single_point,geo_opt}): two
+    atoms in the FCC primitive cell of a = 5.43103 A (set_vec's vectors,
+    set_cell.f90:49-52; symmetry 2, whose minimum image E_SR uses), MT Si
+    (KB, LMAX P: s projector, p local), XC_LDA_XC_TETER93, CUTOFF absent ->
+    80 Ry (the WAVEFUNCTION_CUTOFF key is not read, yamlread.f90:132-138),
+    4 doubly occupied states."""
+    sym, pos = read_xyz(os.path.join(GOLDEN, "structures", "si2_fcc.xyz"))
+    pw = PWSetup(Cell.fcc(5.43103 * ANG2BOHR), 80.0)
+    sSi = psp.CpmdNumericSpecies(os.path.join(GOLDEN, "psp", "Si_MT_LDA.psp"), "Si", 1, True, pw.ecut, pw.ecut_rho)
+    return System(pw, [sSi], [0] * len(sym), pos * ANG2BOHR, 4, np.full(4, 2.0), "TETER93", "si2_ref")
+
+
 def reference_h2_upf():
     """Recorded UPF run (tests/H2_upf): H2 in a 15.1178 bohr cubic cell,
     ONCV H UPF v2 (input LMAX S, LOCAL 0 are not read for UPF species),

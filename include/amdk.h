@@ -59,6 +59,11 @@ typedef struct {
   const int* miller_rho;   /* [3*ngrho] */
   int nbmax;               /* largest band count that will be passed */
   int precision;           /* AMDK_PREC_FP64 or AMDK_PREC_MIXED */
+  int symmetry;            /* lattice code of set_vec (set_cell.f90:17-42: 1 cubic,
+                              2 FCC, 3 BCC, ...), selecting the minimum image of
+                              E_SR (apply_pbc.f90); 0 = not given: cubic when the
+                              cell is cubic, else nearest image in fractional
+                              coordinates */
 } amdk_desc;
 
 /* Context: geometry, maps, FFT plans, device workspaces.
@@ -150,7 +155,7 @@ int amdk_vks(amdk_ctx* ctx, const double* rho_r, int xc, double* v_r, amdk_energ
 
 /* Forces -dE/dR (Ha/bohr) [3*nat]: POTFOR + NLFOR/D_FNL (or UPF_NLFOR/
  * UPF_D_FNL, nuclear_grad.f90:90-199) + E_SR (nuclear_grad.f90:4-372),
- * using rho(G) of the last amdk_vks call. */
+ * using the rho(G) formed last on the device (by amdk_vks or an SCF step). */
 int amdk_forces(amdk_ctx* ctx, const double* c0, int ldc, const double* occ, int nb, double* f);
 
 /* Device-resident SCF (wfn_opt loop body, do_opt.f90:80-129): load C_0,
@@ -163,7 +168,9 @@ int amdk_scf_step(amdk_ctx* ctx, int istep, int xc, amdk_energies* e);
  * this step's state before its wavefunction update: the TFOR branch of
  * wfn_opt (E_SR + POTFOR + NLFOR, do_opt.f90:97-106). */
 int amdk_scf_step_tfor(amdk_ctx* ctx, int istep, int xc, amdk_energies* e, double* f);
-/* Forces for the resident C_0 and the rho(G) of the last SCF step. */
+/* Forces for the resident C_0 and the rho(G) of the last SCF step;
+ * AMDK_ERR_STATE if no step ran since amdk_set_positions or a stateless
+ * amdk_vks has replaced that rho(G) since. */
 int amdk_scf_forces(amdk_ctx* ctx, double* f);
 /* Cholesky-orthonormalise the resident C_0 (ortho_gs(c_0) after a geometry
  * update, molecular_dynamics.f90:108, do_opt.f90:259). */

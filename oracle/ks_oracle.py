@@ -18,6 +18,39 @@ from . import xc_oracle as X
 RAGGIO = 1.2
 
 
+def min_image(d, cell):
+    """Minimum-image difference vector of apply_pbc (apply_pbc.f90:1-60),
+    m = 1: case 1 (simple cubic) folds each component into (-A/2, A/2];
+    case 2 (FCC, A the conventional constant) rotates by 45 degrees about z
+    into the tetragonal frame of the FCC lattice, folds into its box and
+    moves by the body-centre vector when outside |x| + |y| + sqrt2 |z| <
+    2 m_a; cases 3 and above take the nearest image in fractional
+    coordinates (s -= NINT(s))."""
+    mode = cell.pbc_mode()
+    d = np.array(d, np.float64)
+    if mode == 1:
+        A = cell.a[0, 0]
+        d = np.fmod(d, A)
+        return d - A * np.trunc(d / (A / 2.0))
+    if mode == 2:
+        A = math.sqrt(2.0) * np.linalg.norm(cell.a[:, 0])
+        s2 = math.sqrt(2.0)
+        ma = A * s2 / 4.0
+        x, y, z = (d[0] + d[1]) / s2, (-d[0] + d[1]) / s2, d[2]
+        x, y, z = math.fmod(x, 2 * ma), math.fmod(y, 2 * ma), math.fmod(z, 2 * ma * s2)
+        x -= 2 * ma * math.trunc(x / ma)
+        y -= 2 * ma * math.trunc(y / ma)
+        z -= 2 * ma * s2 * math.trunc(z / (s2 * ma))
+        if not (abs(x) + abs(y) + s2 * abs(z) < 2 * ma):
+            x -= math.copysign(ma, x)
+            y -= math.copysign(ma, y)
+            z -= math.copysign(s2 * ma, z)
+        return np.array([(x - y) / s2, (x + y) / s2, z])
+    s = np.linalg.solve(cell.a, d)
+    s = s - np.sign(s) * np.floor(np.abs(s) + 0.5)  # NINT: halves away from zero
+    return cell.a @ s
+
+
 class KSOracle:
     def __init__(self, pw, tables, sp_of_atom, tau, xc="TETER93", workers=1):
         self.pw = pw
@@ -184,23 +217,17 @@ class KSOracle:
 
     def e_sr(self):
         """Short-range ion-ion energy and forces, minimum image, nlp = 0
-        (e_sr, total_energy.f90:133-239; apply_pbc.f90:1-30 for cubic cells)."""
+        (e_sr, total_energy.f90:133-239; min_image restates apply_pbc)."""
         tau = self.tau
         zv = self.T.zv[self.sp]
-        a = self.pw.cell.a
-        L = np.diag(a)
-        cubic = np.allclose(a, np.diag(L)) and np.allclose(L, L[0])
+        cell = self.pw.cell
         rckj = math.sqrt(2.0) * RAGGIO
         esr = 0.0
         f = np.zeros_like(tau)
         nat = len(tau)
         for i in range(nat):
             for j in range(i + 1, nat):
-                d = tau[i] - tau[j]
-                if cubic:
-                    A = L[0]
-                    d = np.fmod(d, A)
-                    d = d - A * np.trunc(d / (A / 2.0))
+                d = min_image(tau[i] - tau[j], cell)
                 r2 = float(d @ d)
                 r = math.sqrt(r2)
                 arg = r / rckj

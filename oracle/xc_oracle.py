@@ -48,28 +48,44 @@ def pz81(rho, sigma=None):
     return lda_x(rho) + pz81_c(rho)
 
 
+# Perdew-Wang 1992 unpolarised correlation, G(rs; A, alpha1, beta1..4), p = 1
+PW92 = dict(A=0.0310907, a1=0.21370, b1=7.5957, b2=3.5876, b3=1.6382, b4=0.49294)
+# PBE 1996
+PBE_KAPPA = 0.804
+PBE_BETA = 0.06672455060314922
+PBE_MU = 0.2195149727645171
+PBE_GAMMA = (1.0 - np.log(2.0)) / np.pi ** 2
+
+
 def pw92_c(rs):
-    A, a1, b1, b2, b3, b4 = 0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294
+    A, a1, b1, b2, b3, b4 = (PW92[k] for k in ("A", "a1", "b1", "b2", "b3", "b4"))
     den = 2.0 * A * (b1 * np.sqrt(rs) + b2 * rs + b3 * rs ** 1.5 + b4 * rs * rs)
     return -2.0 * A * (1.0 + a1 * rs) * np.log(1.0 + 1.0 / den)
 
 
+def pbe_fx(s2):
+    """PBE exchange enhancement F_x(s^2)."""
+    return 1.0 + PBE_KAPPA - PBE_KAPPA / (1.0 + PBE_MU * s2 / PBE_KAPPA)
+
+
+def pbe_h(ec, t2):
+    """PBE gradient correction H(rs, t^2) of the correlation (phi = 1), given
+    eps_c^PW92(rs)."""
+    beta, gamma = PBE_BETA, PBE_GAMMA
+    Ac = (beta / gamma) / (np.exp(-ec / gamma) - 1.0)
+    at2 = Ac * t2
+    return gamma * np.log(1.0 + (beta / gamma) * t2 * (1.0 + at2) / (1.0 + at2 + at2 * at2))
+
+
 def pbe(rho, sigma):
-    kappa, mu = 0.804, 0.2195149727645171
-    beta = 0.06672455060314922
-    gamma = (1.0 - np.log(2.0)) / np.pi ** 2
     kf = (3.0 * np.pi ** 2 * rho) ** (1.0 / 3.0)
     ex_lda = -0.75 * (3.0 / np.pi) ** (1.0 / 3.0) * rho ** (1.0 / 3.0)
     s2 = sigma / (4.0 * kf * kf * rho * rho)
-    fx = 1.0 + kappa - kappa / (1.0 + mu * s2 / kappa)
     rs = _rs(rho)
     ec = pw92_c(rs)
     ks2 = 4.0 * kf / np.pi
     t2 = sigma / (4.0 * ks2 * rho * rho)
-    Ac = (beta / gamma) / (np.exp(-ec / gamma) - 1.0)
-    at2 = Ac * t2
-    H = gamma * np.log(1.0 + (beta / gamma) * t2 * (1.0 + at2) / (1.0 + at2 + at2 * at2))
-    return rho * (ex_lda * fx + ec + H)
+    return rho * (ex_lda * pbe_fx(s2) + ec + pbe_h(ec, t2))
 
 
 FUNCTIONALS = {"TETER93": (teter93, False), "PZ": (pz81, False), "PBE": (pbe, True)}

@@ -137,6 +137,66 @@ def test_pbe_limits_and_derivatives():
         assert abs(d - vs[i]) < 1e-6 * max(1e-3, abs(d))
 
 
+def test_pw92_constants_follow_the_exact_limits():
+    """PW92 fixes A, beta1, beta2 from the exact high-density expansion
+    eps_c = c0 ln rs - c1 + ..., c0 = (1 - ln 2)/pi^2, c1 = 0.046644, and
+    alpha1, beta3, beta4 from the low-density expansion eps_c = -d0/rs +
+    d1/rs^(3/2) + ..., d0 = 0.4335, d1 = 1.4408 (Perdew & Wang, PRB 45,
+    13244 (1992), eqs. 10-12 and Table I).  A mistyped constant breaks one of
+    these relations; the function itself must reach both limits."""
+    P = X.PW92
+    c0 = (1.0 - math.log(2.0)) / math.pi ** 2
+    assert abs(P["A"] / c0 - 1.0) < 1e-5
+    assert abs(-2.0 * P["A"] * math.log(2.0 * P["A"] * P["b1"]) - 0.046644) < 2e-5
+    assert abs(P["b2"] / (2.0 * P["A"] * P["b1"] ** 2) - 1.0) < 1e-4
+    assert abs(P["a1"] / P["b4"] - 0.4335) < 1e-4
+    assert abs(P["a1"] * P["b3"] / P["b4"] ** 2 - 1.4408) < 2e-4
+    # the formula reaches the limits with these constants
+    rs = 1e-10
+    assert abs(X.pw92_c(rs) - P["A"] * math.log(rs) + 0.046644) < 1e-4
+    rs = 1e6
+    assert abs(rs * X.pw92_c(rs) - (-0.4335 + 1.4408 / math.sqrt(rs))) < 1e-4
+    # PBE's gamma is the same exact coefficient
+    assert abs(X.PBE_GAMMA / c0 - 1.0) < 1e-14
+
+
+def test_pw92_agrees_with_pz81_to_fit_accuracy():
+    """Both fit the Ceperley-Alder correlation energies: within 0.6 mHa per
+    electron over rs in [0.5, 10]; a 5 % change of alpha1, beta1 or beta2
+    moves PW92 beyond that (beta3 and beta4 are pinned by the low-density
+    relations above)."""
+    rs = np.linspace(0.5, 10.0, 400)
+    rho = 3.0 / (4.0 * np.pi * rs ** 3)
+    d = X.pw92_c(rs) - X.pz81_c(rho) / rho
+    assert np.abs(d).max() < 6e-4
+    saved = dict(X.PW92)
+    try:
+        for k in ("a1", "b1", "b2"):
+            X.PW92.update(saved)
+            X.PW92[k] = saved[k] * 1.05
+            assert np.abs(X.pw92_c(rs) - X.pz81_c(rho) / rho).max() > 6e-4, k
+    finally:
+        X.PW92.update(saved)
+
+
+def test_pbe_gradient_expansions():
+    """PBE (PRL 77, 3865 (1996)): mu = beta pi^2 / 3 (so that the exchange
+    and correlation gradient terms cancel in the slowly varying limit);
+    F_x = 1 + mu s^2 + O(s^4); H = beta t^2 + O(t^4) at phi = 1; F_x -> 1 +
+    kappa for large s."""
+    assert abs(X.PBE_MU / (X.PBE_BETA * math.pi ** 2 / 3.0) - 1.0) < 1e-14
+    for s2 in (1e-8, 1e-6):
+        fx = X.pbe_fx(s2)
+        assert abs((fx - 1.0) / s2 / X.PBE_MU - 1.0) < 2 * s2 * X.PBE_MU / X.PBE_KAPPA + 1e-7
+    assert abs(X.pbe_fx(1e12) - (1.0 + X.PBE_KAPPA)) < 1e-10
+    for rs in (0.5, 2.0, 6.0):
+        ec = X.pw92_c(rs)
+        for t2 in (1e-8, 1e-6):
+            h = X.pbe_h(ec, t2)
+            assert abs(h / t2 / X.PBE_BETA - 1.0) < 1e-4 * t2 / 1e-6 + 1e-9
+        assert abs(X.pbe_h(ec, 1e12) + ec) < 1e-9  # H -> -eps_c: correlation vanishes
+
+
 def test_smoothing_matches_reference_thresholds():
     r = np.array([-1.0, 5e-10, 1e-9, 2e-9, 4e-8, 4.1e-8])
     smoo, dsmoo, act = X.smooth(r)
@@ -145,6 +205,30 @@ def test_smoothing_matches_reference_thresholds():
 
 
 # -- ions --------------------------------------------------------------------------
+def test_min_image_rules():
+    """apply_pbc restated: the FCC rule (case 2) gives the true minimum
+    image (brute force over lattice translations); the cubic and the
+    fractional rules give an equivalent vector with fractional coordinates
+    in [-1/2, 1/2]; every rule is odd in d."""
+    from oracle.ks_oracle import min_image
+    rng = np.random.default_rng(0)
+    cells = [Cell.fcc(10.263159), Cell.cubic(9.5),
+             Cell(np.array([[10.0, 1.2, 0.0], [0.0, 10.5, 0.9], [0.0, 0.0, 11.0]]))]
+    idx = np.array(np.meshgrid(*[np.arange(-3, 4)] * 3, indexing="ij")).reshape(3, -1)
+    for cell in cells:
+        T = cell.a @ idx
+        for _ in range(400):
+            d = (rng.random(3) - 0.5) * 40.0
+            r = min_image(d, cell)
+            n = np.linalg.solve(cell.a, r - d)
+            assert np.abs(n - np.round(n)).max() < 1e-9  # r - d is a lattice vector
+            np.testing.assert_allclose(min_image(-d, cell), -r, atol=1e-12)
+            if cell.pbc_mode() == 2:
+                assert np.linalg.norm(r) <= np.min(np.linalg.norm(d[:, None] + T, axis=0)) + 1e-9
+            else:
+                assert np.abs(np.linalg.solve(cell.a, r)).max() <= 0.5 + 1e-12
+
+
 def test_structure_factor_and_ewald_direct():
     s = systems.water_box((1, 1, 1), 9.0, 8.0, "PZ", jitter=0.0)
     o = KSOracle(s.pw, s.tables, s.sp, s.tau, s.xc)
@@ -162,6 +246,21 @@ def test_structure_factor_and_ewald_direct():
             e += z[i] * z[j] * math.erfc(r / rc) / r
     assert abs(esr - e) < 1e-14
     assert np.abs(f.sum(0)).max() < 1e-13  # Newton's third law
+
+
+def test_esr_of_recorded_fcc_silicon():
+    """Si2 FCC (tests/bulk_si): the FCC minimum image gives the recorded
+    OVERLAP ENERGY CONTRIBUTION, and its force is the recorded geometry-step-0
+    gradient (the electronic forces of the near-ideal diamond cancel to
+    ~4e-7 Ha/bohr; the pair is counted with one image only, nlp = 0)."""
+    ref = json.load(open(os.path.join(HERE, "golden", "reference_forces.json")))
+    s = systems.reference_si2()
+    assert s.pw.n == (48, 48, 48)
+    o = KSOracle(s.pw, s.tables, s.sp, s.tau, s.xc)
+    esr, f = o.e_sr()
+    assert abs(esr - ref["si2_single_point"]["esr"]) < 1e-10
+    grad = np.array([a["grad"] for a in ref["si2_geo_step0"]["atoms"]])
+    assert np.abs(-f - grad).max() < 1e-6
 
 
 # -- the whole SCF against the reference's recorded run ---------------------------------

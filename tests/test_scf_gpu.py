@@ -130,6 +130,23 @@ def test_ortho_reports_the_failing_band_beyond_the_first_block():
     ctx.close()
 
 
+def test_ortho_reports_the_first_failing_band_of_the_blocked_path():
+    """nb = 200 splits into leaves of 128 and 72 bands; a null band in the
+    first leaf: the later leaf (which runs on the failed factor) must not
+    overwrite the reported minor."""
+    s = SMALL["h2o2_pz"]()
+    nb = 200
+    ctx = KSContext(s.pw, nb, "fp64")
+    c = random_bands(s.pw, nb, seed=19)
+    c[10] = 0.0
+    from amdkiit_amd.lib import AmdkError, ERR_NOT_PD
+    with pytest.raises(AmdkError) as ei:
+        ctx.ortho(c)
+    assert ei.value.code == ERR_NOT_PD
+    assert "leading minor 11)" in str(ei.value)
+    ctx.close()
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_vks_and_energies(sysname, prec):
     s, ctx, orc = build(SMALL[sysname], prec)
@@ -297,11 +314,15 @@ def test_mixed_matches_fp64_at_64_waters_converged():
 
 
 def test_stateless_calls_do_not_disturb_the_resident_state():
-    """amdk_lagrange / amdk_forces with their own occupations between resident
-    SCF steps: the steps continue exactly as without them."""
+    """amdk_lagrange / amdk_vpsi / amdk_density with their own occupations
+    between resident SCF steps: the steps continue exactly as without them.
+    A stateless amdk_vks replaces the step's rho(G), after which
+    amdk_scf_forces refuses instead of using it."""
+    from amdkiit_amd.lib import AmdkError, ERR_STATE
     s = SMALL["h2o2_pz"]()
     occ_other = s.occ.copy()
     occ_other[-1] = 0.0
+    occ_other[0] = 1.0
     runs = []
     for interleave in (False, True):
         ctx = KSContext(s.pw, s.nbands, "fp64")
@@ -314,10 +335,20 @@ def test_stateless_calls_do_not_disturb_the_resident_state():
             es.append(ctx.scf_step(it, s.xc)["etot"])
             if interleave:
                 ctx.lagrange(c, random_bands(s.pw, s.nbands, seed=31), occ_other)
-        runs.append((es, ctx.wfn_store()))
+                ctx.vpsi(c, occ_other)
+                ctx.eval_density(c, occ_other)
+        f_state = ctx.scf_forces()
+        runs.append((es, ctx.wfn_store(), f_state))
+        if interleave:
+            rho = ctx.eval_density(c, occ_other)
+            ctx.vks(rho, s.xc)
+            with pytest.raises(AmdkError) as ei:
+                ctx.scf_forces()
+            assert ei.value.code == ERR_STATE
         ctx.close()
     assert runs[0][0] == runs[1][0]
     assert np.array_equal(runs[0][1], runs[1][1])
+    assert np.array_equal(runs[0][2], runs[1][2])
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
