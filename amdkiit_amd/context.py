@@ -138,6 +138,15 @@ class KSContext:
                                         L.dptr(A.reshape(-1))))
         return A.T.copy()
 
+    def overlap_noalloc(self, ca, cb):
+        """overlap through the allocation-free entry point."""
+        ca, fa = self._flat(ca)
+        cb, fb = self._flat(cb)
+        A = np.zeros((cb.shape[0], ca.shape[0]))
+        self._chk(self.lib.amdk_overlap_noalloc(self.h, L.dptr(fa), L.dptr(fb), ca.shape[1], ca.shape[0], cb.shape[0],
+                                                L.dptr(A.reshape(-1))))
+        return A.T.copy()
+
     def lagrange(self, c0, c2, occ):
         c0, f0 = self._flat(c0)
         c2, f2 = self._flat(c2.copy())

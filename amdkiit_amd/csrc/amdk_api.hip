@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <tuple>
 
@@ -583,6 +584,7 @@ static void check_finite(const double* p, size_t n, const char* what) {
 
 static void alloc_scf_buffers(amdk_ctx* c);
 static void destroy_scf(amdk_ctx* c);
+static double* noalloc_scratch(int dev);
 
 extern "C" {
 
@@ -650,6 +652,8 @@ int amdk_ctx_create(const amdk_desc* d, amdk_ctx** out) {
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw Error(AMDK_ERR_DEVICE, "no HIP device available");
+    AMDK_TRY_HIP(hipGetDevice(&ctx->device));
+    ctx->noalloc_scratch = noalloc_scratch(ctx->device);
     AMDK_TRY_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     AMDK_TRY_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
     for (hipEvent_t* e : {&ctx->ev_fork, &ctx->ev_c0, &ctx->ev_nl})
@@ -884,3 +888,49 @@ int amdk_profile_reset(amdk_ctx* ctx) {
 }  // extern "C"
 
 #include "scf_api.inc"
+
+// ---------------------------------------------------------------------------
+// Allocation-free OVLAP_D.  Its split-K partials live in `scratch`, a fixed
+// array of the code object: every GPU that loads the library holds one copy
+// (32 MiB) until the process ends.  In host code `scratch` names a host-side
+// stand-in, so kernels get the device address that hipGetSymbolAddress
+// returns, looked up once per device (at the first amdk_ctx_create with that
+// device current) and kept in g_scratch_ptr.
+#define AMDK_NOALLOC_SCRATCH_DOUBLES (4u << 20)
+static __device__ __align__(256) double scratch[AMDK_NOALLOC_SCRATCH_DOUBLES];
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[64] = {};
+
+static double* noalloc_scratch(int dev) {
+  if (dev < 0 || dev >= 64) throw Error(AMDK_ERR_DEVICE, "device index outside 0..63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw Error(AMDK_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  return g_scratch_ptr[dev];
+}
+
+extern "C" int amdk_overlap_noalloc(amdk_ctx* ctx, const double* ca, const double* cb, int ldc, int n1, int n2,
+                                    double* A) {
+  API_BEGIN
+  check_nb(ctx, n1);
+  check_nb(ctx, n2);
+  if (!ca || !cb || !A) throw Error(AMDK_ERR_ARG, "null pointer");
+  if ((size_t)n1 * n2 > AMDK_NOALLOC_SCRATCH_DOUBLES)
+    throw Error(AMDK_ERR_ARG, "n1*n2 = " + std::to_string((size_t)n1 * n2) + " exceeds the allocation-free limit of " +
+                                  std::to_string(AMDK_NOALLOC_SCRATCH_DOUBLES) + " (use amdk_overlap)");
+  stage_in(ctx, ca, ldc, n1, ctx->Ct);
+  stage_in(ctx, cb, ldc, n2, ctx->C2);
+  const Sphere& S = ctx->spw;
+  // every split writes all n1*n2 partials it owns before k_ovl_reduce reads
+  // them: nothing in the array has to start at zero
+  ovl_into<double2>(ctx, (const double2*)ctx->Ct, S.ldc, n1, (const double2*)ctx->C2, S.ldc, n2, S.npw, 1.0, ctx->d_S,
+                    n1, nullptr, 0.0, nullptr, false, ctx->noalloc_scratch, AMDK_NOALLOC_SCRATCH_DOUBLES);
+  AMDK_TRY_HIP(hipMemcpyAsync(A, ctx->d_S, sizeof(double) * n1 * n2, hipMemcpyDeviceToHost, ctx->stream));
+  sync(ctx);
+  API_END
+}

@@ -50,6 +50,8 @@ struct amdk_ctx {
   bool fft_generic = false;   // AMDK_FFT=stockham forces the LDS Stockham passes
   int nbmax = 0;
   hipStream_t stream = nullptr;
+  int device = 0;                     // device current at amdk_ctx_create
+  double* noalloc_scratch = nullptr;  // that device's copy of the library's fixed scratch array
   // second stream of the SCF step (work that only reads C0 runs beside the
   // density pass and V_KS) and the events that fork and join it
   hipStream_t side = nullptr;

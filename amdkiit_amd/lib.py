@@ -68,6 +68,7 @@ SIGNATURES = {
     "amdk_fnl": (_i, [_vp, _dp, _i, _i, _dp]),
     "amdk_fnonloc": (_i, [_vp, _dp, _dp, _i, _dp, _i]),
     "amdk_overlap": (_i, [_vp, _dp, _dp, _i, _i, _i, _dp]),
+    "amdk_overlap_noalloc": (_i, [_vp, _dp, _dp, _i, _i, _i, _dp]),
     "amdk_lagrange": (_i, [_vp, _dp, _dp, _i, _dp, _i, _dp, _dp]),
     "amdk_ortho": (_i, [_vp, _dp, _i, _i]),
     "amdk_vks": (_i, [_vp, _dp, _i, _dp, C.POINTER(Energies)]),

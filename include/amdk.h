@@ -129,6 +129,16 @@ int amdk_fnonloc(amdk_ctx* ctx, const double* nl, double* c2, int ldc, const dou
 /* A(i,j) = 2 Re <ca_i|cb_j>_half - ca_i(0) cb_j(0), column-major (n1, n2).
  * Replaces OVLAP_D(NG_PW, N1, N2, A, CA, CB) (wfn_initialize.f90:234-263). */
 int amdk_overlap(amdk_ctx* ctx, const double* ca, const double* cb, int ldc, int n1, int n2, double* A);
+/* amdk_overlap that allocates nothing: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Its split-K partial
+ * sums go to a fixed 32 MiB array of the library, one per GPU, held until the
+ * process ends (looked up at amdk_ctx_create); everything else uses buffers
+ * the context allocated at set-up.  Limit: n1*n2 <= 4,194,304 (AMDK_ERR_ARG
+ * beyond it).  Calls on contexts of the same GPU share that array, so calls
+ * that overlap in time on one GPU (several host threads) race; nothing in it
+ * needs to start at zero. */
+int amdk_overlap_noalloc(amdk_ctx* ctx, const double* ca, const double* cb, int ldc, int n1, int n2, double* A);
 /* Gradient projection: G = <c2|c0> masked by occupation, c2 -= c0 G,
  * Im c2(G=0) = 0; returns GEMAX and CNORM.
  * Replaces FORCES + eval_GMAX (gradient.f90:5-61). */
