@@ -38,25 +38,45 @@ def build_lib(verbose=True):
 
 
 def build_fortran(verbose=True):
-    """Compile the ISO_C_BINDING module and its smoke driver (host code only)."""
+    """Compile the ISO_C_BINDING modules (amdk_hip interfaces, amdk_ref with
+    the reference's routine names) and the two drivers (host code only)."""
     fdir = os.path.join(HERE, "fortran")
-    srcs = [os.path.join(fdir, "amdk_hip.f90"), os.path.join(fdir, "amdk_smoke.f90")]
-    if not all(os.path.exists(s) for s in srcs) or not shutil.which(FC):
+    mods = [os.path.join(fdir, f) for f in ("amdk_hip.f90", "amdk_ref.f90")]
+    progs = {"amdk_smoke": "amdk_smoke.f90", "amdk_wfopt": "amdk_wfopt.f90"}
+    if not all(os.path.exists(s) for s in mods) or not shutil.which(FC):
         return None
-    out = os.path.join(fdir, "amdk_smoke")
     moddir = os.path.join(fdir, "_mod")
     os.makedirs(moddir, exist_ok=True)
-    cmd = [FC, "-O2", "-J", moddir, "-o", out] + srcs + [
-        "-L" + HERE, "-lamdk", "-Wl,-rpath," + HERE]
+    outs = []
+    for exe, src in progs.items():
+        out = os.path.join(fdir, exe)
+        cmd = [FC, "-O2", "-J", moddir, "-o", out] + mods + [os.path.join(fdir, src)] + [
+            "-L" + HERE, "-lamdk", "-Wl,-rpath," + HERE]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        outs.append(out)
+    return outs
+
+
+def build_oracle_ref(verbose=True):
+    """Build the reference's own L-BFGS into oracle/_ref (a checker for the
+    restated one) when the reference tree is present; never on a box
+    without it."""
+    ref = os.environ.get("AMDK_REFERENCE", "/root/reference")
+    if not os.path.isdir(os.path.join(ref, "source")) or not shutil.which("make"):
+        return None
+    cmd = ["make", "-C", os.path.join(ROOT, "oracle"), "REF=" + ref]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    return out
+    return os.path.join(ROOT, "oracle", "_ref", "liblbfgs_ref.so")
 
 
 def main():
     build_lib()
     build_fortran()
+    build_oracle_ref()
 
 
 if __name__ == "__main__":

@@ -585,6 +585,9 @@ static void check_finite(const double* p, size_t n, const char* what) {
 static void alloc_scf_buffers(amdk_ctx* c);
 static void destroy_scf(amdk_ctx* c);
 static double* noalloc_scratch(int dev);
+static void grid_in(amdk_ctx* c, const double* host, double* dev);
+static void grid_out(amdk_ctx* c, const double* dev, double* host);
+static void check_finite_grid(const amdk_ctx* c, const double* host, const char* what);
 
 extern "C" {
 
@@ -607,6 +610,8 @@ int amdk_ctx_create(const amdk_desc* d, amdk_ctx** out) {
     {
       const char* f = std::getenv("AMDK_FFT");
       ctx->fft_generic = f && std::string(f) == "stockham";
+      const char* o = std::getenv("AMDK_OVL_BIG");
+      ctx->ovl_big = !(o && std::string(o) == "0");
     }
     std::memcpy(ctx->acell, d->a_cell, sizeof(ctx->acell));
     // reciprocal: g_j = 2 pi * row j of inv(A), A columns = a_j
@@ -745,9 +750,9 @@ int amdk_set_vks(amdk_ctx* ctx, const double* v_r) {
   API_BEGIN
   if (!v_r) throw Error(AMDK_ERR_ARG, "null v_r");
   const size_t nr = (size_t)ctx->n1 * ctx->n2 * ctx->n3;
-  check_finite(v_r, nr, "v_r");
+  check_finite_grid(ctx, v_r, "v_r");
   double* stage = ctx->grid[1];  // dense scratch grid, free between calls
-  AMDK_TRY_HIP(hipMemcpyAsync(stage, v_r, sizeof(double) * nr, hipMemcpyHostToDevice, ctx->stream));
+  grid_in(ctx, v_r, stage);
   const double s = 1.0 / (double)nr;  // forward FFT normalisation folded into V
   if (ctx->prec == AMDK_PREC_FP64)
     k_scale_copy<double><<<1024, 256, 0, ctx->stream>>>(stage, (double*)ctx->vr, nr, s);
@@ -819,7 +824,7 @@ int amdk_density(amdk_ctx* ctx, const double* c0, int ldc, const double* occ, in
   if (ctx->prec == AMDK_PREC_FP64) run_density<double>(ctx, dc0, nb, drho);
   else run_density<float>(ctx, dc0, nb, drho);
   sync(ctx);
-  download(ctx, rho_r, drho, sizeof(double) * nr);
+  grid_out(ctx, drho, rho_r);
   API_END
 }
 

@@ -48,6 +48,7 @@ struct amdk_ctx {
   amdk::PbcInfo pbc;          // minimum-image rule of E_SR (from amdk_desc.symmetry)
   int prec = AMDK_PREC_FP64;
   bool fft_generic = false;   // AMDK_FFT=stockham forces the LDS Stockham passes
+  bool ovl_big = true;        // 128 x 128 overlap tiles where they fit (AMDK_OVL_BIG=0: 64 x 64 only)
   int nbmax = 0;
   hipStream_t stream = nullptr;
   int device = 0;                     // device current at amdk_ctx_create
@@ -149,6 +150,13 @@ struct amdk_ctx {
   // d_occ, d_wnorm, d_finv and d_wpair hold the resident state's occupations
   // (API calls with their own occupations overwrite them)
   bool occ_state_loaded = false;
+  // caller layouts (amdk_set_layout): real-space grids with leading
+  // dimensions ld1 x ld2 (0: dense n1 x n2), nl(1, nat, nghmax, nb) when
+  // nl_nghmax > 0, force(3, NA_MAX, nsp) when force_na_max > 0
+  int grid_ld1 = 0, grid_ld2 = 0;
+  int nl_nghmax = 0;
+  int force_na_max = 0;
+  std::vector<int> h_idx_in_sp;  // index of each atom among the atoms of its species
   // allocations to free
   std::vector<void*> owned;
   std::string err;

@@ -12,20 +12,21 @@ namespace amdk {
 // A = alpha * (2 sum_s part_s - x0 y0) + beta * A, then A(m, n) = 0 where
 // occ (if given) is zero for band m or n.  A block sums the split-K partials
 // of 64 consecutive outputs with 4 interleaved groups of splits, combined in a
-// fixed order (deterministic).  sym: the partials hold only the 64 x 64 tiles
-// on and above the diagonal (X = Y); an element of an off-diagonal tile sets
-// both A(m, n) and A(n, m), and the elements below skip.
+// fixed order (deterministic).  sym: the partials hold only the tile x tile
+// tiles on and above the diagonal (X = Y); an element of an off-diagonal
+// tile sets both A(m, n) and A(n, m), and the elements below skip.
 template <typename V>
 __global__ void __launch_bounds__(256) k_ovl_reduce(const double* __restrict__ part, int nsplit, int M, int N,
                                                     const V* __restrict__ X, int ldx, const V* __restrict__ Y, int ldy,
                                                     double alpha, double* __restrict__ A, int lda, int g0,
-                                                    double beta, const double* __restrict__ occ, int sym) {
+                                                    double beta, const double* __restrict__ occ, int sym,
+                                                    int tile) {
   __shared__ double sh[4][64];
   const int l = threadIdx.x & 63, grp = threadIdx.x >> 6;
   const size_t MN = (size_t)M * N;
   const size_t o = (size_t)blockIdx.x * 64 + l;
   const int m = (int)(o % M), n = (int)(o / M);
-  const bool live = o < MN && !(sym && m / 64 > n / 64);
+  const bool live = o < MN && !(sym && m / tile > n / tile);
   double s = 0;
   if (live) {
 #pragma unroll 4
@@ -39,7 +40,7 @@ __global__ void __launch_bounds__(256) k_ovl_reduce(const double* __restrict__ p
     const bool empty = occ && (occ[m] == 0.0 || occ[n] == 0.0);  // empty bands take no part
     double* a = A + (size_t)n * lda + m;
     *a = empty ? 0.0 : alpha * v + (beta != 0.0 ? beta * *a : 0.0);
-    if (sym && m / 64 < n / 64) {
+    if (sym && m / tile < n / tile) {
       double* at = A + (size_t)m * lda + n;
       *at = empty ? 0.0 : alpha * v + (beta != 0.0 ? beta * *at : 0.0);
     }

@@ -171,6 +171,117 @@ __global__ void __launch_bounds__(256) k_ovl_mfma(const double* __restrict__ X, 
       }
 }
 
+// The same product with a 128 x 128 block tile: 8 waves in 2 (m) x 4 (n),
+// each 64 x 32 (4 x 2 MFMA tiles), 32-deep K tiles staged in LDS with the
+// next one prefetched into registers.  Every operand row a block loads feeds
+// twice the output of the 64 x 64 tile, which halves the operand traffic
+// from L2 per flop.  Otherwise as k_ovl_mfma (split-K partials, XCD-aware
+// block order, sym: tiles on and above the diagonal, diagonal tiles load
+// their rows once; FP32 tiles flushed into FP64 every 32 K).
+constexpr int OVB_T = 128, OVB_KT = 32;
+template <typename TC>
+__global__ void __launch_bounds__(512) k_ovl_big(const double* __restrict__ X, int ldx, int M,
+                                                 const double* __restrict__ Y, int ldy, int N, int K,
+                                                 int kchunk, int nsplit, int per, int sym,
+                                                 double* __restrict__ part) {
+  typedef typename Acc4<TC>::type AccT;
+  constexpr bool F64 = sizeof(TC) == 8;
+  constexpr int KH = OVB_KT / 2;                       // double2 per row and tile
+  constexpr int LD = (OVB_T * KH) / 512;               // double2 loads per thread and operand
+  __shared__ TC xs[OVB_T][OVB_KT + 1];
+  __shared__ TC ys[OVB_T][OVB_KT + 1];
+  const int tm = (M + OVB_T - 1) / OVB_T, tn = (N + OVB_T - 1) / OVB_T;
+  const int tiles = sym ? tn * (tn + 1) / 2 : tm * tn;
+  const int L = xcd_block(blockIdx.x, per);
+  if (L >= tiles * nsplit) return;
+  const int split = L / tiles, t = L % tiles;
+  int m0, n0;
+  if (sym) {  // t = j (j + 1) / 2 + i, i <= j
+    int j = 0;
+    while ((j + 1) * (j + 2) / 2 <= t) ++j;
+    m0 = (t - j * (j + 1) / 2) * OVB_T;
+    n0 = j * OVB_T;
+  } else {
+    m0 = (t % tm) * OVB_T;
+    n0 = (t / tm) * OVB_T;
+  }
+  const int kb = split * kchunk, ke = min(K, kb + kchunk);
+  const bool diag = sym && m0 == n0;  // uniform per block
+  const TC(*yt)[OVB_KT + 1] = diag ? xs : ys;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  dbl4 acc[4][2];
+  AccT tac[4][2];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      acc[a][b] = (dbl4){0.0, 0.0, 0.0, 0.0};
+      tac[a][b] = (AccT){0, 0, 0, 0};
+    }
+  double2 px[LD], py[LD];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int it = 0; it < LD; ++it) {
+      const int e = threadIdx.x + 512 * it, r = e / KH, k = k0 + 2 * (e % KH);
+      // K and every chunk start are even, so a pair never straddles ke
+      px[it] = (m0 + r < M && k < ke) ? *(const double2*)(X + (size_t)(m0 + r) * ldx + k) : make_double2(0.0, 0.0);
+      if (!diag)
+        py[it] = (n0 + r < N && k < ke) ? *(const double2*)(Y + (size_t)(n0 + r) * ldy + k) : make_double2(0.0, 0.0);
+    }
+  };
+  load(kb);
+  for (int k0 = kb; k0 < ke; k0 += OVB_KT) {
+#pragma unroll
+    for (int it = 0; it < LD; ++it) {
+      const int e = threadIdx.x + 512 * it, r = e / KH, c = 2 * (e % KH);
+      xs[r][c] = (TC)px[it].x;
+      xs[r][c + 1] = (TC)px[it].y;
+      if (!diag) {
+        ys[r][c] = (TC)py[it].x;
+        ys[r][c + 1] = (TC)py[it].y;
+      }
+    }
+    __syncthreads();
+    if (k0 + OVB_KT < ke) load(k0 + OVB_KT);
+#pragma unroll
+    for (int ks = 0; ks < OVB_KT; ks += 4) {
+      const int kk = ks + lk;
+      TC av[4], bv[2];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) av[a] = xs[wm + 16 * a + li][kk];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) bv[b] = yt[wn + 16 * b + li][kk];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          if constexpr (F64) acc[a][b] = mfma16(av[a], bv[b], acc[a][b]);
+          else tac[a][b] = mfma16(av[a], bv[b], tac[a][b]);
+        }
+    }
+    if constexpr (!F64) {  // FP64 sum of every 32-deep FP32 tile
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc_flush(acc[a][b], tac[a][b]);
+    }
+    __syncthreads();
+  }
+  double* P = part + (size_t)split * M * N;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int m = m0 + wm + 16 * a + acc_row<TC>(lk, v);
+        const int n = n0 + wn + 16 * b + li;
+        if (m < M && n < N) P[(size_t)n * M + m] = acc[a][b][v];
+      }
+}
+
 constexpr int UPM_MC = 32, UPM_NB = 128, UPM_K = 64;
 
 // Block: 64 reals of k (4 waves x 16) x 128 columns n (8 MFMA tiles per

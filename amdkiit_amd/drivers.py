@@ -116,107 +116,48 @@ def bomd(provider, tau0, masses_au, v0, dt_fs, nsteps, log=None, outputs=None, s
     return h, drift, tau, v
 
 
-def lbfgs(provider, x0, m=5, gtol=1e-4, maxiter=20, log=None, outputs=None, symbols=None):
-    """L-BFGS on E(R) with the two-loop recursion and M = 5 corrections, as the
-    reference's geom_opt_lbfgs (do_opt.f90:197-266) calls lbfgs.f.  The step
-    length comes from a strong-Wolfe line search (c1 = 1e-4, c2 = 0.9, Nocedal &
-    Wright alg. 3.5/3.6) instead of lbfgs.f's Moré-Thuente routine.  With
-    `outputs` and `symbols`, every accepted geometry (the first included) is
-    appended to atom_coord.xyz and atom_force.dat, the latter holding the
-    gradient dE/dR as the reference's PRINT_COORDINATE call does
-    (do_opt.f90:244-249)."""
-    x = np.array(x0, np.float64).reshape(-1)
+def lbfgs(provider, x0, m=5, gtol=1e-4, maxiter=100, log=None, outputs=None, symbols=None):
+    """Geometry optimisation as geom_opt_lbfgs (do_opt.f90:197-266) runs it:
+    evaluate E and F at x, print the geometry when the previous LBFGS call
+    completed an iteration (the first geometry included), hand (x, E, -F) to
+    the reverse-communication L-BFGS of amdkiit_amd.lbfgs (M = 5, XTOL =
+    1e-16, EPS = gtol on the 2-norm of the gradient), which either asks for
+    the next geometry or stops; at most maxiter printed steps.  Note that a
+    printed geometry is the first trial point of a line search (the point
+    LBFGS asks for right after completing an iteration), not necessarily the
+    accepted one.
+
+    Returns (x, hist, accepted): x the last accepted geometry (the start if
+    none), hist [(E, max|g|, |g|)] of the printed geometries, accepted the
+    same for every accepted iterate.  With `outputs` and `symbols`, each
+    printed geometry goes to atom_coord.xyz and its gradient dE/dR to
+    atom_force.dat (PRINT_COORDINATE, do_opt.f90:244-249)."""
+    from .lbfgs import LBFGS
+
     shape = np.shape(x0)
-
-    def fg(xx):
-        e, f = provider(xx.reshape(shape))
-        return e, -np.asarray(f, np.float64).reshape(-1)
-
-    def record(iopt, e, xx, g):
-        if outputs is not None:
-            outputs.atom_coord(iopt, e, symbols, xx.reshape(-1, 3))
-            outputs.atom_force(iopt, e, symbols, g.reshape(-1, 3))
-
-    e, g = fg(x)
-    S, Y = [], []
-    hist = [(e, float(np.abs(g).max()))]
-    record(0, e, x, g)
-    for it in range(1, maxiter + 1):
-        if np.abs(g).max() < gtol:
+    x = np.array(x0, np.float64).reshape(-1)
+    opt = LBFGS(x.size, m=m, eps=gtol, xtol=1e-16)
+    hist, accepted = [], []
+    x_acc = x.copy()
+    print_next, iopt = True, 0
+    while True:
+        e, f = provider(x.reshape(shape))
+        g = -np.asarray(f, np.float64).reshape(-1)
+        rec = (e, float(np.abs(g).max()), float(np.sqrt(g @ g)))
+        if print_next:
+            if outputs is not None:
+                outputs.atom_coord(iopt, e, symbols, x.reshape(-1, 3))
+                outputs.atom_force(iopt, e, symbols, g.reshape(-1, 3))
+            hist.append(rec)
+            if log:
+                log(iopt, e, rec[1])
+            iopt += 1
+        x_eval = x.copy()
+        iflag = opt.step(x, e, g)
+        if opt.accepted:
+            accepted.append(rec)
+            x_acc = x_eval
+        print_next = opt.accepted
+        if iflag <= 0 or iopt > maxiter:
             break
-        q = g.copy()
-        al = []
-        for s_, y_ in reversed(list(zip(S, Y))):
-            r = 1.0 / float(y_ @ s_)
-            a = r * float(s_ @ q)
-            al.append((a, r, s_, y_))
-            q -= a * y_
-        if S:
-            q *= float(S[-1] @ Y[-1]) / float(Y[-1] @ Y[-1])
-        else:
-            q *= min(1.0, 0.1 / max(np.abs(g).max(), 1e-12))
-        for a, r, s_, y_ in reversed(al):
-            b = r * float(y_ @ q)
-            q += (a - b) * s_
-        d = -q
-        if float(d @ g) >= 0:
-            d = -g
-        step, e1, g1 = _wolfe(fg, x, e, g, d)
-        s_ = step * d
-        y_ = g1 - g
-        x = x + s_
-        e, g = e1, g1
-        if float(y_ @ s_) > 1e-12:
-            S.append(s_)
-            Y.append(y_)
-            if len(S) > m:
-                S.pop(0)
-                Y.pop(0)
-        hist.append((e, float(np.abs(g).max())))
-        record(it, e, x, g)
-        if log:
-            log(it, e, float(np.abs(g).max()))
-    return x.reshape(shape), hist
-
-
-def _wolfe(fg, x, e0, g0, d, c1=1e-4, c2=0.9, amax=4.0, maxev=12):
-    dg0 = float(g0 @ d)
-    a_prev, e_prev, dg_prev = 0.0, e0, dg0
-    a = 1.0
-    for i in range(maxev):
-        e, g = fg(x + a * d)
-        dg = float(g @ d)
-        if e > e0 + c1 * a * dg0 or (i > 0 and e >= e_prev):
-            return _zoom(fg, x, e0, dg0, d, a_prev, e_prev, dg_prev, a, e, dg, c1, c2)
-        if abs(dg) <= -c2 * dg0:
-            return a, e, g
-        if dg >= 0:
-            return _zoom(fg, x, e0, dg0, d, a, e, dg, a_prev, e_prev, dg_prev, c1, c2)
-        a_prev, e_prev, dg_prev = a, e, dg
-        a = min(2.0 * a, amax)
-    return a, e, g
-
-
-def _zoom(fg, x, e0, dg0, d, alo, elo, dglo, ahi, ehi, dghi, c1, c2, maxev=12):
-    for _ in range(maxev):
-        # cubic interpolation between the bracket ends, safeguarded to the middle 80 %
-        d1 = dglo + dghi - 3.0 * (elo - ehi) / (alo - ahi)
-        rad = d1 * d1 - dglo * dghi
-        if rad > 0:
-            d2 = math.copysign(math.sqrt(rad), ahi - alo)
-            a = ahi - (ahi - alo) * (dghi + d2 - d1) / (dghi - dglo + 2 * d2)
-        else:
-            a = 0.5 * (alo + ahi)
-        lo, hi = min(alo, ahi), max(alo, ahi)
-        a = min(max(a, lo + 0.1 * (hi - lo)), hi - 0.1 * (hi - lo))
-        e, g = fg(x + a * d)
-        dg = float(g @ d)
-        if e > e0 + c1 * a * dg0 or e >= elo:
-            ahi, ehi, dghi = a, e, dg
-        else:
-            if abs(dg) <= -c2 * dg0:
-                return a, e, g
-            if dg * (ahi - alo) >= 0:
-                ahi, ehi, dghi = alo, elo, dglo
-            alo, elo, dglo = a, e, dg
-    return a, e, g
+    return x_acc.reshape(shape), hist, accepted

@@ -1,11 +1,7 @@
-! amdk_hip: ISO_C_BINDING interface to libamdk (include/amdk.h) and drop-in
-! shims with the argument lists of the reference's hot-path subroutines.
-!
-! The reference keeps its call sites; each shim is called from the body of the
-! routine it replaces, passing the module globals the routine used implicitly
-! (NGPW_l, the context handle).  A non-zero status prints amdk_last_error and
-! STOPs, which is what the reference does on LAPACK failures
-! (wfn_initialize.f90:513-527).
+! amdk_hip: ISO_C_BINDING interfaces of libamdk (include/amdk.h) and the
+! error check.  The reference's routines with their own names and argument
+! lists (VPSI, FNONLOC, fnl, OVLAP_D, FORCES, PCG, ortho_gs, ...) are in
+! module amdk_ref (amdk_ref.f90), built on these interfaces.
 module amdk_hip
   use iso_c_binding
   implicit none
@@ -30,6 +26,12 @@ module amdk_hip
     type(c_ptr) :: zv, vps, rhos, nproj, proj_l, twnl, dmat
   end type
 
+  type, bind(C), public :: amdk_layout
+    integer(c_int) :: grid_ld1, grid_ld2, nl_nghmax, force_na_max
+  end type
+
+  integer(c_int), parameter, public :: AMDK_FORCE_ESR = 1, AMDK_FORCE_POTFOR = 2, AMDK_FORCE_NLFOR = 4
+
   type, bind(C), public :: amdk_energies
     real(c_double) :: etot, ekin, estat, eself, esr, eloc, enl, exc, eh
     real(c_double) :: gemax, cnorm, nelec
@@ -40,8 +42,8 @@ module amdk_hip
             amdk_overlap, amdk_lagrange, amdk_ortho, amdk_vks, amdk_forces, &
             amdk_wfn_load, amdk_wfn_store, amdk_scf_step, amdk_scf_forces, amdk_scf_step_tfor, &
             amdk_wfn_ortho
-  public :: amdk_check, vpsi_hip, fnonloc_hip, fnl_hip, ovlap_d_hip, forces_hip, ortho_gs_hip, &
-            eval_density_hip
+  public :: amdk_check, amdk_set_layout, amdk_set_positions_atco, amdk_forces_add, amdk_pcg, amdk_ekin, &
+            amdk_enl, amdk_ion_energies, amdk_overlap_noalloc
 
   interface
     integer(c_int) function amdk_ctx_create(d, ctx) bind(C, name="amdk_ctx_create")
@@ -182,6 +184,62 @@ module amdk_hip
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: f(*)
     end function
+    integer(c_int) function amdk_overlap_noalloc(ctx, ca, cb, ldc, n1, n2, a) bind(C, name="amdk_overlap_noalloc")
+      import :: c_int, c_ptr, c_double_complex, c_double
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: ca(*), cb(*)
+      integer(c_int), value :: ldc, n1, n2
+      real(c_double), intent(out) :: a(*)
+    end function
+    integer(c_int) function amdk_set_layout(ctx, l) bind(C, name="amdk_set_layout")
+      import :: c_int, c_ptr, amdk_layout
+      type(c_ptr), value :: ctx
+      type(amdk_layout), intent(in) :: l
+    end function
+    integer(c_int) function amdk_set_positions_atco(ctx, nsp, atom_p_sp, na_max, atco) &
+        bind(C, name="amdk_set_positions_atco")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsp, na_max
+      integer(c_int), intent(in) :: atom_p_sp(*)
+      real(c_double), intent(in) :: atco(*)
+    end function
+    integer(c_int) function amdk_forces_add(ctx, c0, ldc, occ, nb, parts, f) bind(C, name="amdk_forces_add")
+      import :: c_int, c_ptr, c_double_complex, c_double
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: c0(*)
+      integer(c_int), value :: ldc, nb, parts
+      real(c_double), intent(in) :: occ(*)
+      real(c_double), intent(inout) :: f(*)
+    end function
+    integer(c_int) function amdk_pcg(ctx, c0, c2, hnm1, ldc, occ, nb, istep) bind(C, name="amdk_pcg")
+      import :: c_int, c_ptr, c_double_complex, c_double
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(inout) :: c0(*), hnm1(*)
+      complex(c_double_complex), intent(in) :: c2(*)
+      integer(c_int), value :: ldc, nb, istep
+      real(c_double), intent(in) :: occ(*)
+    end function
+    integer(c_int) function amdk_ekin(ctx, c0, ldc, occ, nb, ekin) bind(C, name="amdk_ekin")
+      import :: c_int, c_ptr, c_double_complex, c_double
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: c0(*)
+      integer(c_int), value :: ldc, nb
+      real(c_double), intent(in) :: occ(*)
+      real(c_double), intent(out) :: ekin
+    end function
+    integer(c_int) function amdk_enl(ctx, nl, occ, nb, enl) bind(C, name="amdk_enl")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: nl(*), occ(*)
+      integer(c_int), value :: nb
+      real(c_double), intent(out) :: enl
+    end function
+    integer(c_int) function amdk_ion_energies(ctx, esr, eself) bind(C, name="amdk_ion_energies")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: esr, eself
+    end function
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
       import :: c_size_t, c_ptr
       type(c_ptr), value :: s
@@ -202,72 +260,6 @@ contains
     call c_f_pointer(amdk_last_error(ctx), msg, [n])
     write (*, '(A,A,I3,A,*(A))') where, ': amdk error ', rc, ' : ', msg(1:n)
     stop 1
-  end subroutine
-
-  ! VPSI(C_ATOM, C2_nl, FC, NOSTATE)       wfn_initialize.f90:268-365
-  subroutine vpsi_hip(ctx, ngpw, c_atom, c2_nl, fc, nostate)
-    type(c_ptr), intent(in) :: ctx
-    integer, intent(in) :: ngpw, nostate
-    complex(c_double_complex), intent(in) :: c_atom(ngpw, nostate)
-    complex(c_double_complex), intent(inout) :: c2_nl(ngpw, nostate)
-    real(c_double), intent(in) :: fc(nostate)
-    call amdk_check(ctx, amdk_vpsi(ctx, c_atom, int(ngpw, c_int), c2_nl, fc, int(nostate, c_int)), 'VPSI')
-  end subroutine
-
-  ! FNONLOC(CA, NOSTATE, FC) with the nl of the last fnl   wfn_initialize.f90:560-612
-  subroutine fnonloc_hip(ctx, ngpw, ca, nostate, fc, nl)
-    type(c_ptr), intent(in) :: ctx
-    integer, intent(in) :: ngpw, nostate
-    complex(c_double_complex), intent(inout) :: ca(ngpw, nostate)
-    real(c_double), intent(in) :: fc(nostate), nl(*)
-    call amdk_check(ctx, amdk_fnonloc(ctx, nl, ca, int(ngpw, c_int), fc, int(nostate, c_int)), 'FNONLOC')
-  end subroutine
-
-  ! fnl(C0, nstat) -> nl(nproj, nstat)        pseudopotential.f90:164-226
-  subroutine fnl_hip(ctx, ngpw, c0, nstat, nl)
-    type(c_ptr), intent(in) :: ctx
-    integer, intent(in) :: ngpw, nstat
-    complex(c_double_complex), intent(in) :: c0(ngpw, nstat)
-    real(c_double), intent(out) :: nl(*)
-    call amdk_check(ctx, amdk_fnl(ctx, c0, int(ngpw, c_int), int(nstat, c_int), nl), 'fnl')
-  end subroutine
-
-  ! OVLAP_D(NG_PW, N1, N2, A, CA, CB)          wfn_initialize.f90:234-263
-  subroutine ovlap_d_hip(ctx, ng_pw, n1, n2, a, ca, cb)
-    type(c_ptr), intent(in) :: ctx
-    integer, intent(in) :: ng_pw, n1, n2
-    real(c_double), intent(out) :: a(n1, n2)
-    complex(c_double_complex), intent(in) :: ca(ng_pw, n1), cb(ng_pw, n2)
-    call amdk_check(ctx, amdk_overlap(ctx, ca, cb, int(ng_pw, c_int), int(n1, c_int), int(n2, c_int), a), 'OVLAP_D')
-  end subroutine
-
-  ! FORCES + eval_GMAX                         gradient.f90:5-61
-  subroutine forces_hip(ctx, ngpw, c0, c2, occ, nstate, gemax, cnorm)
-    type(c_ptr), intent(in) :: ctx
-    integer, intent(in) :: ngpw, nstate
-    complex(c_double_complex), intent(in) :: c0(ngpw, nstate)
-    complex(c_double_complex), intent(inout) :: c2(ngpw, nstate)
-    real(c_double), intent(in) :: occ(nstate)
-    real(c_double), intent(out) :: gemax, cnorm
-    call amdk_check(ctx, amdk_lagrange(ctx, c0, c2, int(ngpw, c_int), occ, int(nstate, c_int), gemax, cnorm), 'FORCES')
-  end subroutine
-
-  ! ortho_gs(CP)                                wfn_initialize.f90:448-558
-  subroutine ortho_gs_hip(ctx, ngpw, cp, nstate)
-    type(c_ptr), intent(in) :: ctx
-    integer, intent(in) :: ngpw, nstate
-    complex(c_double_complex), intent(inout) :: cp(ngpw, nstate)
-    call amdk_check(ctx, amdk_ortho(ctx, cp, int(ngpw, c_int), int(nstate, c_int)), 'ortho_gs')
-  end subroutine
-
-  ! eval_density                                density.f90:5-98
-  subroutine eval_density_hip(ctx, ngpw, c0, occ, nstate, rho)
-    type(c_ptr), intent(in) :: ctx
-    integer, intent(in) :: ngpw, nstate
-    complex(c_double_complex), intent(in) :: c0(ngpw, nstate)
-    real(c_double), intent(in) :: occ(nstate)
-    real(c_double), intent(out) :: rho(*)
-    call amdk_check(ctx, amdk_density(ctx, c0, int(ngpw, c_int), occ, int(nstate, c_int), rho), 'eval_density')
   end subroutine
 
 end module amdk_hip

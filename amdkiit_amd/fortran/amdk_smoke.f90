@@ -67,7 +67,7 @@ program amdk_smoke
   v = v0
   call amdk_check(ctx, amdk_set_vks(ctx, v), 'set_vks')
   c2 = (0.0d0, 0.0d0)
-  call vpsi_hip(ctx, ng, c, c2, occ, nb)
+  call amdk_check(ctx, amdk_vpsi(ctx, c, int(ng, c_int), c2, occ, int(nb, c_int)), 'VPSI')
   err = 0.0d0
   do b = 1, nb
     x = 0.5d0 * occ(b)
@@ -81,8 +81,8 @@ program amdk_smoke
     write (*, '(A)') 'FAIL'
     stop 1
   end if
-  call ortho_gs_hip(ctx, ng, c, nb)
-  call ovlap_d_hip(ctx, ng, nb, nb, s, c, c)
+  call amdk_check(ctx, amdk_ortho(ctx, c, int(ng, c_int), int(nb, c_int)), 'ortho_gs')
+  call amdk_check(ctx, amdk_overlap(ctx, c, c, int(ng, c_int), int(nb, c_int), int(nb, c_int), s), 'OVLAP_D')
   err = 0.0d0
   do b = 1, nb
     s(b, b) = s(b, b) - 1.0d0

@@ -41,6 +41,13 @@ class Species(C.Structure):
     ]
 
 
+class Layout(C.Structure):
+    _fields_ = [("grid_ld1", C.c_int), ("grid_ld2", C.c_int), ("nl_nghmax", C.c_int), ("force_na_max", C.c_int)]
+
+
+FORCE_ESR, FORCE_POTFOR, FORCE_NLFOR = 1, 2, 4
+
+
 class Energies(C.Structure):
     _fields_ = [(n, C.c_double) for n in
                 ("etot", "ekin", "estat", "eself", "esr", "eloc", "enl", "exc", "eh", "gemax", "cnorm", "nelec")]
@@ -82,6 +89,13 @@ SIGNATURES = {
     "amdk_bands_to_dev": (_i, [_vp, _dp, _i, _i, _vp]),
     "amdk_bands_from_dev": (_i, [_vp, _vp, _i, _dp, _i]),
     "amdk_sphere_info": (_i, [_vp, _i, _ip]),
+    "amdk_set_layout": (_i, [_vp, C.POINTER(Layout)]),
+    "amdk_set_positions_atco": (_i, [_vp, _i, _ip, _i, _dp]),
+    "amdk_forces_add": (_i, [_vp, _dp, _i, _dp, _i, _i, _dp]),
+    "amdk_pcg": (_i, [_vp, _dp, _dp, _dp, _i, _dp, _i, _i]),
+    "amdk_ekin": (_i, [_vp, _dp, _i, _dp, _i, _dp]),
+    "amdk_enl": (_i, [_vp, _dp, _dp, _i, _dp]),
+    "amdk_ion_energies": (_i, [_vp, _dp, _dp]),
     "amdk_profile_enable": (_i, [_vp, _i]),
     "amdk_profile_names": (_i, [_vp, C.c_char_p, _i]),
     "amdk_profile_query": (_i, [_vp, C.c_char_p, _dp, _ip]),

@@ -30,7 +30,13 @@ HPSI_KERNELS = ("rows_inv", "cols_inv", "z_vpsi", "cols_fwd", "rows_fwd_hpsi")
 CONFIGS = {
     "h2o_32_pbe": "32 H2O PBE 70 Ry, 18.63 bohr, 128 bands, 100^3",
     "h2o_64_pbe": "64 H2O PBE 80 Ry, 23.47 bohr, 256 bands, 144^3",
+    "h2o_128_pbe": "128 H2O PBE 80 Ry, 29.57 bohr, 512 bands, 180^3",
+    "h2o_32_ref": "32 H2O of the reference's recorded run: MT O/H, Teter93, 80 Ry, 18.61 bohr, 128 bands, 108^3",
 }
+# The reference's recorded run of that system: SCF step times (ms) in the
+# last column of tests/32_water/wf_opt/SP_ENERGY.dat, 48 MPI ranks.
+REF32_S_PER_STEP = 0.597
+REF32_NOTE = "tests/32_water/wf_opt/SP_ENERGY.dat: ~597 ms per SCF step on 48 MPI ranks (its step-time column)"
 
 
 def log(*a):
@@ -188,27 +194,55 @@ def gemm_table(prof, pw, nb, nproj, prec, nsteps):
     return out
 
 
-def cpu_baseline(s, c0, seconds_hint):
-    """FP64 numpy/scipy restatement of the same SCF step on the host cores."""
+def cpu_baseline(s, c0, seconds_hint, cores=None):
+    """FP64 numpy/scipy restatement of the same SCF step on the host cores
+    (all of them, or `cores` threads for FFT and BLAS)."""
+    from threadpoolctl import threadpool_limits
+
     from oracle.ks_oracle import KSOracle
 
-    ncores = os.cpu_count() or 1
-    orc = KSOracle(s.pw, s.tables, s.sp, s.tau, s.xc, workers=ncores)
-    c = c0.copy()
-    t0 = time.perf_counter()
-    c, _ = orc.scf_step(c, s.occ, 1)  # warm-up (plans, caches)
-    t1 = time.perf_counter()
-    n = 0
-    while True:
-        c, _ = orc.scf_step(c, s.occ, 2 + n)
-        n += 1
-        if time.perf_counter() - t1 > seconds_hint or n >= 3:
-            break
-    dt = (time.perf_counter() - t1) / n
+    ncores = cores or os.cpu_count() or 1
+    with threadpool_limits(limits=ncores):
+        orc = KSOracle(s.pw, s.tables, s.sp, s.tau, s.xc, workers=ncores)
+        c = c0.copy()
+        t0 = time.perf_counter()
+        c, _ = orc.scf_step(c, s.occ, 1)  # warm-up (plans, caches)
+        t1 = time.perf_counter()
+        n = 0
+        while True:
+            c, _ = orc.scf_step(c, s.occ, 2 + n)
+            n += 1
+            if time.perf_counter() - t1 > seconds_hint or n >= 3:
+                break
+        dt = (time.perf_counter() - t1) / n
     return {"value": 1.0 / dt, "unit": "SCF steps/s", "cores": ncores, "kind": "port",
             "sample": f"{n} SCF step(s) of the same workload after 1 warm-up step, FP64 numpy/scipy "
-                      f"(scipy.fft workers={ncores}, threaded BLAS)",
+                      f"(scipy.fft workers={ncores}, BLAS threads={ncores})",
             "s_per_step": dt, "warmup_s": t1 - t0}
+
+
+def measure_secondary(cfg_name, sysobj, steps, warmup, seed):
+    """Mixed and FP64 steps/s of another workload from one seeded start."""
+    s2 = sysobj
+    rng2 = np.random.default_rng(seed)
+    c02 = (rng2.random((s2.nbands, s2.pw.ngpw)) - 0.5 + 1j * (rng2.random((s2.nbands, s2.pw.ngpw)) - 0.5)) / (1.0 + s2.pw.g2_pw)
+    c02[:, 0] = c02[:, 0].real
+    cx, d2, e2, p2, _ = run_gpu(s2, "mixed", steps, warmup, c02, profile_steps=2)
+    hp2 = sum(p2[k][0] / p2[k][1] for k in HPSI_KERNELS if k in p2 and p2[k][1])
+    hb2 = hpsi_bytes(s2.pw, s2.nbands, "mixed")
+    g2 = gemm_table(p2, s2.pw, s2.nbands, cx.nproj, "mixed", 2)
+    cx.close()
+    c64, d64, e64b, p64, _ = run_gpu(s2, "fp64", steps, warmup, c02, profile_steps=1)
+    c64.close()
+    return {"workload": cfg_name, "steps": steps, "grid": list(s2.pw.n), "ngpw": s2.pw.ngpw, "bands": s2.nbands,
+            "mixed_steps_per_s": steps / d2, "mixed_ms_per_step": d2 / steps * 1e3,
+            "fp64_steps_per_s": steps / d64, "fp64_ms_per_step": d64 / steps * 1e3, "mixed_speedup": d64 / d2,
+            "hpsi_ms": hp2, "hpsi_GBps": hb2 / (hp2 * 1e-3) / 1e9,
+            "hpsi_frac_hbm": hb2 / (hp2 * 1e-3) / 1e9 / HBM_PEAK,
+            "etot_diff_per_atom": abs(e2["etot"] - e64b["etot"]) / s2.nat,
+            "mixed_ms_by_region": {k: v[0] / 2 for k, v in sorted(p2.items(), key=lambda x: -x[1][0])},
+            "mixed_gemm_mfma": g2,
+            "fp64_ms_by_region": {k: v[0] for k, v in sorted(p64.items(), key=lambda x: -x[1][0])}}
 
 
 def run_md(cfg, prec, nsteps, dt_fs=0.5, temperature=300.0, gcut=1e-6):
@@ -257,8 +291,8 @@ def run_lbfgs(cfg, prec, nsteps, gcut=3e-5):
     c = ctx.ortho(c0)
     prov = D.GPUForceProvider(ctx, s, c, gcut=gcut, maxit=6000)
     t0 = time.perf_counter()
-    _, hist = D.lbfgs(prov, s.tau.copy(), gtol=1e-4, maxiter=nsteps,
-                      log=lambda i, e, fm: log(f"[bench]   lbfgs {i} E {e:.8f} max|g| {fm:.2e}"))
+    _, hist, _ = D.lbfgs(prov, s.tau.copy(), gtol=1e-4, maxiter=nsteps,
+                         log=lambda i, e, fm: log(f"[bench]   lbfgs {i} E {e:.8f} max|g| {fm:.2e}"))
     wall = time.perf_counter() - t0
     ctx.close()
     return {"config": cfg, "precision": prec, "lbfgs_steps": len(hist) - 1, "scf_gcut": gcut,
@@ -350,32 +384,24 @@ def main():
         log(f"[bench] fp64: {fp64['ms_per_step']:.3f} ms/step; mixed speedup {fp64['mixed_speedup']:.2f}x; "
             f"|dE|/atom={fp64['etot_diff_per_atom']:.2e}")
 
-    secondary = None
+    secondary = sec128 = ref32 = None
     if not a.no_secondary and a.config == "h2o_32_pbe":
         log("[bench] secondary: 64 H2O")
-        s2 = systems.CONFIGS["h2o_64_pbe"]()
-        rng2 = np.random.default_rng(2025)
-        c02 = (rng2.random((s2.nbands, s2.pw.ngpw)) - 0.5 + 1j * (rng2.random((s2.nbands, s2.pw.ngpw)) - 0.5)) / (1.0 + s2.pw.g2_pw)
-        c02[:, 0] = c02[:, 0].real
-        st2 = max(5, a.steps // 2)
-        cx, d2, e2, p2, _ = run_gpu(s2, "mixed", st2, a.warmup, c02, profile_steps=2)
-        hp2 = sum(p2[k][0] / p2[k][1] for k in HPSI_KERNELS if k in p2 and p2[k][1])
-        hb2 = hpsi_bytes(s2.pw, s2.nbands, "mixed")
-        g2 = gemm_table(p2, s2.pw, s2.nbands, cx.nproj, "mixed", 2)
-        cx.close()
-        c64, d64, e64b, p64, _ = run_gpu(s2, "fp64", st2, a.warmup, c02, profile_steps=1)
-        c64.close()
-        secondary = {"workload": CONFIGS["h2o_64_pbe"], "steps": st2,
-                     "mixed_steps_per_s": st2 / d2, "mixed_ms_per_step": d2 / st2 * 1e3,
-                     "fp64_steps_per_s": st2 / d64, "mixed_speedup": d64 / d2,
-                     "hpsi_ms": hp2, "hpsi_GBps": hb2 / (hp2 * 1e-3) / 1e9,
-                     "hpsi_frac_hbm": hb2 / (hp2 * 1e-3) / 1e9 / HBM_PEAK,
-                     "etot_diff_per_atom": abs(e2["etot"] - e64b["etot"]) / s2.nat,
-                     "mixed_ms_by_region": {k: v[0] / 2 for k, v in sorted(p2.items(), key=lambda x: -x[1][0])},
-                     "mixed_gemm_mfma": g2,
-                     "fp64_ms_by_region": {k: v[0] for k, v in sorted(p64.items(), key=lambda x: -x[1][0])}}
+        secondary = measure_secondary(CONFIGS["h2o_64_pbe"], systems.CONFIGS["h2o_64_pbe"](),
+                                      max(5, a.steps // 2), a.warmup, 2025)
         log(f"[bench] 64 H2O: mixed {secondary['mixed_ms_per_step']:.2f} ms/step, fp64 speedup "
             f"{secondary['mixed_speedup']:.2f}x, H|psi> {secondary['hpsi_GBps']:.0f} GB/s")
+        log("[bench] secondary: 128 H2O")
+        sec128 = measure_secondary(CONFIGS["h2o_128_pbe"], systems.CONFIGS["h2o_128_pbe"](), 5, 2, 2026)
+        log(f"[bench] 128 H2O: mixed {sec128['mixed_ms_per_step']:.2f} ms/step, fp64 speedup "
+            f"{sec128['mixed_speedup']:.2f}x, H|psi> {sec128['hpsi_GBps']:.0f} GB/s")
+        log("[bench] secondary: the reference's recorded 32 H2O run (MT/Teter93, 80 Ry, 108^3)")
+        ref32 = measure_secondary(CONFIGS["h2o_32_ref"], systems.reference_water(32), 10, a.warmup, 2027)
+        ref32["reference_s_per_step"] = REF32_S_PER_STEP
+        ref32["reference_run"] = REF32_NOTE
+        ref32["vs_reference_run"] = REF32_S_PER_STEP / (ref32["mixed_ms_per_step"] * 1e-3)
+        log(f"[bench] ref 32 H2O: mixed {ref32['mixed_ms_per_step']:.3f} ms/step, fp64 "
+            f"{ref32['fp64_ms_per_step']:.3f} ms/step (recorded reference run {REF32_S_PER_STEP} s/step)")
 
     md = None
     if a.md_steps > 0:
@@ -399,6 +425,9 @@ def main():
         log("[bench] cpu baseline (FP64 port) ...")
         cpu = cpu_baseline(s, c0, a.cpu_seconds)
         log(f"[bench] cpu: {cpu['s_per_step']:.2f} s/step on {cpu['cores']} threads")
+        one = cpu_baseline(s, c0, 1.0, cores=1)
+        log(f"[bench] cpu: {one['s_per_step']:.2f} s/step on 1 thread")
+        cpu["one_core"] = {k: one[k] for k in ("value", "unit", "cores", "kind", "sample", "s_per_step")}
 
     out = {
         "metric": METRIC,
@@ -430,6 +459,8 @@ def main():
         "energy": {"etot": e["etot"], "gemax": e["gemax"], "nelec": e["nelec"], "after_steps": it},
         "fp64_gpu": fp64,
         "secondary_64h2o": secondary,
+        "secondary_128h2o": sec128,
+        "secondary_reference_32h2o": ref32,
         "cpu_baseline": cpu,
     }
     if md:

@@ -13,7 +13,8 @@
  *    is at [2*(b*ldc + g)], i.e. Fortran COMPLEX*16 C(LDC, NB).
  *  - Plane-wave order is the caller's: the Miller indices passed to
  *    amdk_ctx_create fix it (any half sphere; G and -G must not both appear).
- *  - Real-space grids are dense n1*n2*n3 doubles, x fastest (no padding).
+ *  - Real-space grids are n1*n2*n3 doubles, x fastest, dense unless
+ *    amdk_set_layout gives the reference's padded leading dimensions.
  *  - Return value 0 = success; otherwise an AMDK_ERR_* code and
  *    amdk_last_error() describes it.  The Fortran shims turn a non-zero
  *    status into STOP, as the reference does on LAPACK failures.
@@ -195,6 +196,57 @@ int amdk_bands_from_dev(amdk_ctx* ctx, const void* src, int nb, double* c, int l
 /* Sphere geometry as the FFT kernels see it: which = 0 wavefunction sphere,
  * 1 density sphere; out = {npw, ldc, half rows, full rows, planes, R}. */
 int amdk_sphere_info(const amdk_ctx* ctx, int which, int* out6);
+
+/* ---- the reference's own array layouts and the remaining routines of one
+ * SCF step, for a caller that keeps the reference's globals on the host ---- */
+
+/* Layouts of the caller's host arrays, applied from this call on to every
+ * entry point that takes or returns them.  0 keeps the dense default.
+ *  grid_ld1, grid_ld2: real-space grids (RHO, V_POT) with element (x, y, z)
+ *    at x + ld1 (y + ld2 z), the reference's nrlead padding (RHO(NNR1),
+ *    NNR1 = nrlead(1) nrlead(2) nrlead(3), init_module.f90:159,
+ *    gvectors.f90:64,494-497); padding entries are neither read nor written.
+ *  nl_nghmax: nl(1, atom_t, nghmax, nstate) (pseudopotential.f90:174):
+ *    projector k of atom a (atoms in amdk_set_positions order, grouped by
+ *    species as the reference's isa) at a + nat (k + nghmax b).
+ *  force_na_max: force(3, NA_MAX, sp_t) (system_data_types.f90:151): atom a,
+ *    the ia-th of its species s, at 3 (ia + NA_MAX s) + k; unused atom slots
+ *    are zeroed where a routine overwrites the array. */
+typedef struct {
+  int grid_ld1, grid_ld2;
+  int nl_nghmax;
+  int force_na_max;
+} amdk_layout;
+int amdk_set_layout(amdk_ctx* ctx, const amdk_layout* layout);
+
+/* exp_igrxfactor (exp_igr.f90:4-140) from the reference's own arrays:
+ * atco(3, NA_MAX, sp_t) bohr with atom_p_sp(sp_t) atoms per species;
+ * otherwise as amdk_set_positions (atoms in species order). */
+int amdk_set_positions_atco(amdk_ctx* ctx, int nsp, const int* atom_p_sp, int na_max, const double* atco);
+
+/* Parts of the nuclear forces, ADDED into f (the caller's force layout), as
+ * E_SR (total_energy.f90:133-239, whose caller zeroes force first), POTFOR
+ * (nuclear_grad.f90:4-50, rho(G) of the last amdk_vks or SCF step) and NLFOR
+ * / UPF_NLFOR (nuclear_grad.f90:51-89,153-199, for c0 with occupations occ)
+ * accumulate into force.  c0/occ/nb are only read for NLFOR. */
+enum { AMDK_FORCE_ESR = 1, AMDK_FORCE_POTFOR = 2, AMDK_FORCE_NLFOR = 4 };
+int amdk_forces_add(amdk_ctx* ctx, const double* c0, int ldc, const double* occ, int nb, int parts, double* f);
+
+/* PCG(istep) (cg.f90:63-145 with hess/PRECON, cg.f90:3-61,281-311):
+ * P = f^-1 K(G)^-1 C2, d = P + gamma d_prev (gamma = forget ||P||^2 /
+ * ||P_prev||^2, forget 0.6 / 0.91 / 0.95 for istep <= 10 / <= 20 / later),
+ * c0 += (5^2 / 400) d.  hnm1 is HNM1 (d_prev in, d out; not read at istep 1).
+ * The ||P||^2 history is kept by the context, as the reference SAVEs it, and
+ * apart from the resident SCF's. */
+int amdk_pcg(amdk_ctx* ctx, double* c0, const double* c2, double* hnm1, int ldc, const double* occ, int nb, int istep);
+
+/* Energy terms that e_total (total_energy.f90:4-37) sums besides those of
+ * amdk_vks: E_kin = sum_i f_i sum_G |G|^2 |c_i(G)|^2 (e_ke, total_energy.f90:38-63),
+ * E_nl of pp_energy (pseudopotential.f90:55-99) from nl (caller layout),
+ * and E_SR / E_self of the positions (total_energy.f90:133-239). */
+int amdk_ekin(amdk_ctx* ctx, const double* c0, int ldc, const double* occ, int nb, double* ekin);
+int amdk_enl(amdk_ctx* ctx, const double* nl, const double* occ, int nb, double* enl);
+int amdk_ion_energies(amdk_ctx* ctx, double* esr, double* eself);
 
 /* Per-kernel timing with HIP events on the context stream. */
 int amdk_profile_enable(amdk_ctx* ctx, int on);

@@ -69,6 +69,15 @@ def main(ref):
     tab, a, b = energy_table(os.path.join(t, "bulk_si", "single_point", "amdkiit.out"))
     tab["cite"] = f"tests/bulk_si/single_point/amdkiit.out:{a}-{b}"
     data["si2_single_point"] = tab
+    # end of the Si2 geometry optimisation: final energy table and the last
+    # LB1 line (ITER, NFN = energy/gradient evaluations)
+    path = os.path.join(t, "bulk_si", "geo_opt", "amdkiit.out")
+    tab, a, b = energy_table(path)
+    lines = open(path).read().split("\n")
+    k = max(i for i, l in enumerate(lines) if re.match(r"\s*\d+\s+\d+\s+-?\d\.\d+D[+-]\d+", l))
+    it, nfn = (int(v) for v in lines[k].split()[:2])
+    tab.update({"cite": f"tests/bulk_si/geo_opt/amdkiit.out:{k + 1},{a}-{b}", "iterations": it, "evaluations": nfn})
+    data["si2_geo_final"] = tab
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_forces.json")
     with open(out, "w") as f:
         json.dump(data, f, indent=1)

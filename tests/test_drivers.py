@@ -66,7 +66,7 @@ def test_lbfgs_minimises_quadratic_and_rosenbrock():
         d = x - xs
         return 0.5 * float(d @ H @ d), -(H @ d)
 
-    x, hist = D.lbfgs(quad, np.zeros(6), gtol=1e-8, maxiter=60)
+    x, hist, acc = D.lbfgs(quad, np.zeros(6), gtol=1e-8, maxiter=60)
     assert np.abs(x - xs).max() < 1e-6
 
     def rosen(x):
@@ -75,9 +75,11 @@ def test_lbfgs_minimises_quadratic_and_rosenbrock():
         g = np.array([-2 * (1 - a) - 400 * a * (b - a * a), 200 * (b - a * a)])
         return e, -g
 
-    x, hist = D.lbfgs(rosen, np.array([-1.2, 1.0]), gtol=1e-6, maxiter=200)
+    x, hist, acc = D.lbfgs(rosen, np.array([-1.2, 1.0]), gtol=1e-6, maxiter=200)
     assert np.abs(x - 1.0).max() < 1e-4
-    assert all(hist[i + 1][0] <= hist[i][0] + 1e-12 for i in range(len(hist) - 1))
+    # the line search's sufficient-decrease test: accepted energies decrease
+    assert all(acc[i + 1][0] <= acc[i][0] + 1e-12 for i in range(len(acc) - 1))
+    assert acc[-1][2] <= 1e-6
 
 
 def test_drivers_write_the_reference_output_files(tmp_path):
@@ -103,12 +105,22 @@ def test_drivers_write_the_reference_output_files(tmp_path):
     assert np.abs(last[:, :3] * ANGS2BOHR - tau).max() < 1e-5
     assert np.abs(last[:, 3:] * ANGS2BOHR - v1).max() < 1e-5
     out = RunOutputs(str(tmp_path / "opt"))
-    x, hist = D.lbfgs(prov, x0, gtol=1e-6, maxiter=30, outputs=out, symbols=["O", "H", "H"])
+    seen = []
+
+    def prov_rec(t):
+        seen.append(np.array(t))
+        return prov(t)
+
+    x, hist, acc = D.lbfgs(prov_rec, x0, gtol=1e-6, maxiter=30, outputs=out, symbols=["O", "H", "H"])
     fr = (tmp_path / "opt" / "atom_force.dat").read_text().splitlines()
     assert len(fr) == 5 * len(hist)
     e_last = float(fr[-4].split()[1])
     g_last = np.array([[float(v) for v in ln.split()[1:]] for ln in fr[-3:]])
-    e, f = prov(x)
-    assert abs(e_last - e) < 1e-14 and np.abs(g_last + f).max() < 1e-14  # gradient = -F
     co = (tmp_path / "opt" / "atom_coord.xyz").read_text().splitlines()
-    assert np.abs(np.array([[float(v) for v in ln.split()[1:]] for ln in co[-3:]]) - x * 0.529177).max() < 1e-14
+    x_last = np.array([[float(v) for v in ln.split()[1:]] for ln in co[-3:]]) / 0.529177
+    # the last printed geometry is one of the evaluated ones, with its E and dE/dR
+    k = int(np.argmin([np.abs(t - x_last).max() for t in seen]))
+    assert np.abs(seen[k] - x_last).max() < 1e-5
+    e, f = prov(seen[k])
+    assert abs(e_last - e) < 1e-14 and np.abs(g_last + f).max() < 1e-14  # gradient = -F
+    assert abs(hist[-1][0] - e) < 1e-14

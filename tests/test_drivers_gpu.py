@@ -64,8 +64,8 @@ def test_bomd_energy_error_is_second_order_in_dt():
 def test_lbfgs_lowers_energy_and_forces():
     s, ctx, c0 = water_pair()
     prov = D.GPUForceProvider(ctx, s, c0, gcut=1e-7)
-    x, hist = D.lbfgs(prov, s.tau.copy(), gtol=1e-3, maxiter=4,
-                      log=lambda *a: print("opt", *a, flush=True))
-    assert hist[-1][0] < hist[0][0]
-    assert hist[-1][1] < hist[0][1]
+    x, hist, acc = D.lbfgs(prov, s.tau.copy(), gtol=1e-3, maxiter=4,
+                           log=lambda *a: print("opt", *a, flush=True))
+    assert acc[-1][0] < hist[0][0]
+    assert acc[-1][2] < hist[0][2]
     ctx.close()

@@ -99,7 +99,7 @@ def test_ortho_rejects_dependent_bands():
     ctx.close()
 
 
-@pytest.mark.parametrize("nb", [1, 37, 64, 65, 128, 129, 200, 300])
+@pytest.mark.parametrize("nb", [1, 37, 64, 65, 128, 129, 200, 300, 512])
 def test_ortho_band_counts(nb):
     """Cholesky orthonormalisation at band counts around the one-workgroup
     limit (128) and the blocked split: U^-1 against the oracle, C^T C = 1."""
@@ -280,6 +280,38 @@ def test_large_context_tables_and_first_step(prec):
     assert abs(e["enl"] - orc.e_nl(orc.fnl(c), s.occ)) < 1e-6 * abs(e["enl"])
     assert abs(e["nelec"] - s.occ.sum()) < 1e-10 * s.occ.sum()
     ctx.close()
+
+
+def test_128_waters_tables_first_steps_mixed_vs_fp64():
+    """128 H2O PBE 80 Ry (180^3, 156,277 plane waves, 512 bands, the largest
+    configuration): nl against the oracle, E_kin against its closed form, and
+    three SCF steps in both precisions from one start (1e-6 Ha/atom)."""
+    s = systems.CONFIGS["h2o_128_pbe"]()
+    print(f"case=h2o_128 n={s.pw.n} npw={s.pw.ngpw} nb={s.nbands}", flush=True)
+    c0 = random_bands(s.pw, s.nbands, seed=22)
+    orc = KSOracle(s.pw, s.tables, s.sp, s.tau, s.xc, workers=8)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        ctx = KSContext(s.pw, s.nbands, prec)
+        ctx.set_species(s.tables)
+        ctx.set_positions(s.sp, s.tau)
+        c = ctx.ortho(c0)
+        if prec == "fp64":
+            S = orc.ovlap(c[:64], c[:64])
+            assert np.abs(S - np.eye(64)).max() < 1e-11
+            assert rel(ctx.fnl(c), orc.fnl(c)) < TOL[prec]
+            ekin = float(np.sum(s.occ[:, None] * s.pw.g2_pw[None, :] * np.abs(c) ** 2))
+        ctx.wfn_load(c, s.occ)
+        es = [ctx.scf_step(it, s.xc) for it in (1, 2, 3)]
+        if prec == "fp64":
+            assert abs(es[0]["ekin"] - ekin) < 1e-10 * ekin
+        for e in es:
+            assert abs(e["nelec"] - s.occ.sum()) < 1e-10 * s.occ.sum()
+        res[prec] = es
+        print(f"  {prec}: " + " ".join(f"{e['etot']:.8f}" for e in es), flush=True)
+        ctx.close()
+    for a, b in zip(res["mixed"], res["fp64"]):
+        assert abs(a["etot"] - b["etot"]) / s.nat < 1e-6
 
 
 def test_mixed_matches_fp64_at_64_waters_converged():
