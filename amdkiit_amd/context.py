@@ -17,7 +17,7 @@ class KSContext:
         self.lib = L.load()
         self.pw = pw
         self.nbmax = int(nbmax)
-        self.precision = L.PREC_FP64 if precision == "fp64" else L.PREC_MIXED
+        self.precision = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED, "bf16": L.PREC_BF16}[precision]
         d = L.Desc()
         d.n1, d.n2, d.n3 = pw.n
         a = np.asfortranarray(pw.cell.a)  # columns a_j ; a_cell[3*j+i] = (a_j)_i
@@ -199,6 +199,10 @@ class KSContext:
         fp = L.dptr(f) if want_forces else None
         self._chk(self.lib.amdk_scf_step_tfor(self.h, istep, L.XC[xc], C.byref(e), fp))
         return e.as_dict(), (f.reshape(-1, 3) if want_forces else None)
+
+    def set_ortho_method(self, method):
+        """'cholesky' (ortho_gs, default) or 'lowdin' for the SCF step."""
+        self._chk(self.lib.amdk_set_ortho_method(self.h, {"cholesky": 0, "lowdin": 1}[method]))
 
     def ortho_resident(self):
         self._chk(self.lib.amdk_wfn_ortho(self.h))

@@ -598,7 +598,8 @@ int amdk_ctx_create(const amdk_desc* d, amdk_ctx** out) {
   int rc = AMDK_OK;
   try {
     if (!d) throw Error(AMDK_ERR_ARG, "null descriptor");
-    if (d->precision != AMDK_PREC_FP64 && d->precision != AMDK_PREC_MIXED) throw Error(AMDK_ERR_ARG, "bad precision mode");
+    if (d->precision != AMDK_PREC_FP64 && d->precision != AMDK_PREC_MIXED && d->precision != AMDK_PREC_BF16)
+      throw Error(AMDK_ERR_ARG, "bad precision mode");
     if (d->nbmax < 1) throw Error(AMDK_ERR_ARG, "nbmax must be >= 1");
     if (d->n1 > 192 || d->n2 > 192 || d->n3 > 192) throw Error(AMDK_ERR_ARG, "grid dimension above 192 is not supported");
     ctx->n1 = d->n1; ctx->n2 = d->n2; ctx->n3 = d->n3;

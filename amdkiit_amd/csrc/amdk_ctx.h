@@ -49,6 +49,7 @@ struct amdk_ctx {
   int prec = AMDK_PREC_FP64;
   bool fft_generic = false;   // AMDK_FFT=stockham forces the LDS Stockham passes
   bool ovl_big = true;        // 128 x 128 overlap tiles where they fit (AMDK_OVL_BIG=0: 64 x 64 only)
+  bool ortho_lowdin = false;  // SCF step orthonormalises by C S^-1/2 instead of Cholesky
   int nbmax = 0;
   hipStream_t stream = nullptr;
   int device = 0;                     // device current at amdk_ctx_create

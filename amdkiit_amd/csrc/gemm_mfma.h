@@ -282,6 +282,134 @@ __global__ void __launch_bounds__(512) k_ovl_big(const double* __restrict__ X, i
       }
 }
 
+// BF16 arithmetic with one refinement term (the BF16 recipe of the
+// projector and Lagrange-multiplier products): every FP64 operand x is split
+// into hi = bf16(x) and lo = bf16(x - hi), and a product is formed as
+// hi*hi' + (hi*lo' + lo*hi') on v_mfma_f32_16x16x32_bf16, summed in FP32 over
+// one 32-deep K tile and in FP64 across tiles.  The dropped lo*lo' and the
+// rounding of lo leave ~2^-16 of each term (FP32-level sums need 2^-24).
+// 16x16x32 operands (wave64): lane l holds A(l % 16, 8 (l / 16) + j) and
+// B(8 (l / 16) + j, l % 16), j = 0..7; the result is spread as the FP32
+// 16x16x4 form's (D(4 (l / 16) + v, l % 16)).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ flt4 mfma_bf16(bf16x8 a, bf16x8 b, flt4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ void split_bf16(double x, __bf16& hi, __bf16& lo) {
+  hi = (__bf16)(float)x;
+  lo = (__bf16)(float)(x - (double)(float)hi);
+}
+
+// Overlap partials (as k_ovl_big: 128 x 128 tile, 8 waves of 64 x 32,
+// split-K, sym) in the BF16 recipe above.  32-deep K tiles; hi and lo planes
+// of both operands in LDS with a 40-element row pitch (80 B: the 16-byte
+// reads of the 16 rows of an MFMA operand fall on distinct bank groups).
+constexpr int OVH_KT = 32, OVH_P = OVH_KT + 8;
+__global__ void __launch_bounds__(512) k_ovl_bf16(const double* __restrict__ X, int ldx, int M,
+                                                  const double* __restrict__ Y, int ldy, int N, int K,
+                                                  int kchunk, int nsplit, int per, int sym,
+                                                  double* __restrict__ part) {
+  constexpr int KH = OVH_KT / 2;
+  constexpr int LD = (OVB_T * KH) / 512;
+  __shared__ __attribute__((aligned(16))) __bf16 xh[OVB_T][OVH_P];
+  __shared__ __attribute__((aligned(16))) __bf16 xl[OVB_T][OVH_P];
+  __shared__ __attribute__((aligned(16))) __bf16 yh[OVB_T][OVH_P];
+  __shared__ __attribute__((aligned(16))) __bf16 yl[OVB_T][OVH_P];
+  const int tm = (M + OVB_T - 1) / OVB_T, tn = (N + OVB_T - 1) / OVB_T;
+  const int tiles = sym ? tn * (tn + 1) / 2 : tm * tn;
+  const int L = xcd_block(blockIdx.x, per);
+  if (L >= tiles * nsplit) return;
+  const int split = L / tiles, t = L % tiles;
+  int m0, n0;
+  if (sym) {
+    int j = 0;
+    while ((j + 1) * (j + 2) / 2 <= t) ++j;
+    m0 = (t - j * (j + 1) / 2) * OVB_T;
+    n0 = j * OVB_T;
+  } else {
+    m0 = (t % tm) * OVB_T;
+    n0 = (t / tm) * OVB_T;
+  }
+  const int kb = split * kchunk, ke = min(K, kb + kchunk);
+  const bool diag = sym && m0 == n0;
+  const __bf16(*yth)[OVH_P] = diag ? xh : yh;
+  const __bf16(*ytl)[OVH_P] = diag ? xl : yl;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  dbl4 acc[4][2];
+  flt4 tac[4][2];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      acc[a][b] = (dbl4){0.0, 0.0, 0.0, 0.0};
+      tac[a][b] = (flt4){0.f, 0.f, 0.f, 0.f};
+    }
+  double2 px[LD], py[LD];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int it = 0; it < LD; ++it) {
+      const int e = threadIdx.x + 512 * it, r = e / KH, k = k0 + 2 * (e % KH);
+      px[it] = (m0 + r < M && k < ke) ? *(const double2*)(X + (size_t)(m0 + r) * ldx + k) : make_double2(0.0, 0.0);
+      if (!diag)
+        py[it] = (n0 + r < N && k < ke) ? *(const double2*)(Y + (size_t)(n0 + r) * ldy + k) : make_double2(0.0, 0.0);
+    }
+  };
+  load(kb);
+  for (int k0 = kb; k0 < ke; k0 += OVH_KT) {
+#pragma unroll
+    for (int it = 0; it < LD; ++it) {
+      const int e = threadIdx.x + 512 * it, r = e / KH, c = 2 * (e % KH);
+      split_bf16(px[it].x, xh[r][c], xl[r][c]);
+      split_bf16(px[it].y, xh[r][c + 1], xl[r][c + 1]);
+      if (!diag) {
+        split_bf16(py[it].x, yh[r][c], yl[r][c]);
+        split_bf16(py[it].y, yh[r][c + 1], yl[r][c + 1]);
+      }
+    }
+    __syncthreads();
+    if (k0 + OVH_KT < ke) load(k0 + OVH_KT);
+    const int kk = 8 * lk;
+    bf16x8 ah[4], al[4], bh[2], bl[2];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      ah[a] = *(const bf16x8*)&xh[wm + 16 * a + li][kk];
+      al[a] = *(const bf16x8*)&xl[wm + 16 * a + li][kk];
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      bh[b] = *(const bf16x8*)&yth[wn + 16 * b + li][kk];
+      bl[b] = *(const bf16x8*)&ytl[wn + 16 * b + li][kk];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        tac[a][b] = mfma_bf16(ah[a], bl[b], tac[a][b]);
+        tac[a][b] = mfma_bf16(al[a], bh[b], tac[a][b]);
+        tac[a][b] = mfma_bf16(ah[a], bh[b], tac[a][b]);
+      }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc_flush(acc[a][b], tac[a][b]);
+    __syncthreads();
+  }
+  double* P = part + (size_t)split * M * N;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int m = m0 + wm + 16 * a + 4 * lk + v;
+        const int n = n0 + wn + 16 * b + li;
+        if (m < M && n < N) P[(size_t)n * M + m] = acc[a][b][v];
+      }
+}
+
 constexpr int UPM_MC = 32, UPM_NB = 128, UPM_K = 64;
 
 // Block: 64 reals of k (4 waves x 16) x 128 columns n (8 MFMA tiles per
@@ -391,6 +519,108 @@ __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X1,
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int n = n0 + 16 * t + acc_row<TC>(lk, v);
+        if (n < N) {
+          double* y = Y + (size_t)n * ldy + k;
+          *y = (zero_g0_im && k == 1) ? 0.0 : (beta != 0.0 ? beta * *y : 0.0) + acc[t][v];
+        }
+      }
+  }
+}
+
+// k_upd_mfma in the BF16 recipe (hi*hi' + hi*lo' + lo*hi' on
+// v_mfma_f32_16x16x32_bf16, one 32-row chunk of m per instruction, FP64
+// sums across chunks).  The chunk's X (32 m x 64 k) and Q (32 m x 128 n) are
+// staged transposed, m fastest, as hi and lo planes, so that each lane reads
+// its 8 consecutive m of an operand in one 16-byte load.
+constexpr int UPH_P = UPM_MC + 8;
+template <bool TRI>
+__global__ void __launch_bounds__(256) k_upd_bf16(const double* __restrict__ X1, int ldx, int M1,
+                                                  const double* __restrict__ Q1, int ldq1, double a1,
+                                                  const double* __restrict__ X2, int M2,
+                                                  const double* __restrict__ Q2, int ldq2, double a2,
+                                                  double* __restrict__ Y, int ldy, int N, int K,
+                                                  double beta, int per, int zero_g0_im) {
+  constexpr int XL = (UPM_MC * UPM_K / 2) / 256;
+  constexpr int QL = (UPM_MC * UPM_NB / 2) / 256;
+  __shared__ __attribute__((aligned(16))) __bf16 qh[UPM_NB][UPH_P];
+  __shared__ __attribute__((aligned(16))) __bf16 ql[UPM_NB][UPH_P];
+  __shared__ __attribute__((aligned(16))) __bf16 xh[UPM_K][UPH_P];
+  __shared__ __attribute__((aligned(16))) __bf16 xl[UPM_K][UPH_P];
+  const int nnb = (N + UPM_NB - 1) / UPM_NB, nkb = (K + UPM_K - 1) / UPM_K;
+  const int L = xcd_block(blockIdx.x, per);
+  if (L >= nnb * nkb) return;
+  const int k0 = (L / nnb) * UPM_K, n0 = (L % nnb) * UPM_NB;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int kl = wave * 16 + li, k = k0 + kl;
+  dbl4 acc[8];
+  flt4 tac[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    acc[t] = (dbl4){0.0, 0.0, 0.0, 0.0};
+    tac[t] = (flt4){0.f, 0.f, 0.f, 0.f};
+  }
+  const int M1p = (M1 + UPM_MC - 1) / UPM_MC * UPM_MC;
+  const int M = M1p + M2;
+  double2 px[XL], pq[QL];
+  auto load = [&](int mb) {
+    const bool two = mb >= M1p;
+    const double* X = two ? X2 : X1;
+    const double* Q = two ? Q2 : Q1;
+    const int ldq = two ? ldq2 : ldq1, Ms = two ? M2 : M1, m0 = two ? mb - M1p : mb;
+#pragma unroll
+    for (int it = 0; it < XL; ++it) {
+      const int e = threadIdx.x + 256 * it, r = e / (UPM_K / 2), kk = k0 + 2 * (e % (UPM_K / 2));
+      px[it] = (m0 + r < Ms && kk < K) ? *(const double2*)(X + (size_t)(m0 + r) * ldx + kk) : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int it = 0; it < QL; ++it) {
+      const int e = threadIdx.x + 256 * it, j = e / (UPM_MC / 2), i = 2 * (e % (UPM_MC / 2));
+      const bool ok = n0 + j < N;
+      const double* q = Q + (size_t)(n0 + j) * ldq + m0 + i;
+      pq[it] = make_double2(ok && m0 + i < Ms ? q[0] : 0.0, ok && m0 + i + 1 < Ms ? q[1] : 0.0);
+    }
+  };
+  load(0);
+  for (int mb = 0; mb < M; mb += UPM_MC) {
+#pragma unroll
+    for (int it = 0; it < XL; ++it) {
+      const int e = threadIdx.x + 256 * it, r = e / (UPM_K / 2), c = 2 * (e % (UPM_K / 2));
+      split_bf16(px[it].x, xh[c][r], xl[c][r]);
+      split_bf16(px[it].y, xh[c + 1][r], xl[c + 1][r]);
+    }
+    const double qa = mb >= M1p ? a2 : a1;
+#pragma unroll
+    for (int it = 0; it < QL; ++it) {
+      const int e = threadIdx.x + 256 * it, j = e / (UPM_MC / 2), i = 2 * (e % (UPM_MC / 2));
+      split_bf16(qa * pq[it].x, qh[j][i], ql[j][i]);
+      split_bf16(qa * pq[it].y, qh[j][i + 1], ql[j][i + 1]);
+    }
+    __syncthreads();
+    if (mb + UPM_MC < M) load(mb + UPM_MC);
+    // tiles t < tskip: columns n0 + 16 t + 15 < mb, Q1 = 0 on this chunk
+    const int tskip = (TRI && mb < M1p) ? min(8, max(0, (mb - n0) / 16)) : 0;
+    const bf16x8 bhv = *(const bf16x8*)&xh[kl][8 * lk];
+    const bf16x8 blv = *(const bf16x8*)&xl[kl][8 * lk];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      if (TRI && t < tskip) continue;
+      const bf16x8 ahv = *(const bf16x8*)&qh[16 * t + li][8 * lk];
+      const bf16x8 alv = *(const bf16x8*)&ql[16 * t + li][8 * lk];
+      tac[t] = mfma_bf16(ahv, blv, tac[t]);
+      tac[t] = mfma_bf16(alv, bhv, tac[t]);
+      tac[t] = mfma_bf16(ahv, bhv, tac[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc_flush(acc[t], tac[t]);
+    __syncthreads();
+  }
+  if (k < K) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int n = n0 + 16 * t + 4 * lk + v;
         if (n < N) {
           double* y = Y + (size_t)n * ldy + k;
           *y = (zero_g0_im && k == 1) ? 0.0 : (beta != 0.0 ? beta * *y : 0.0) + acc[t][v];
@@ -605,6 +835,21 @@ __global__ void __launch_bounds__(64 * NT) k_chol_inv(const double* __restrict__
   }
   const int zr = nz - n;
   for (int e = tid; e < zr * n; e += 64 * NT) X[n + e % zr + (size_t)(e / zr) * ldx] = 0.0;
+}
+
+// W <- 1.5 I - 0.5 W (n x n, leading dimension ld): the factor T_k of the
+// Newton-Schulz iteration for S^-1/2.
+__global__ void k_ns_factor(double* __restrict__ W, int n, int ld) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += gridDim.x * blockDim.x) {
+    const int i = e % n, j = e / n;
+    double* w = W + i + (size_t)j * ld;
+    *w = (i == j ? 1.5 : 0.0) - 0.5 * *w;
+  }
+}
+// W <- I (n x n)
+__global__ void k_identity(double* __restrict__ W, int n, int ld) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += gridDim.x * blockDim.x)
+    W[e % n + (size_t)(e / n) * ld] = (e % n == e / n) ? 1.0 : 0.0;
 }
 
 // C = alpha op(A) op(B) + beta C for the small band-block products of the

@@ -8,7 +8,7 @@ module amdk_hip
   private
 
   integer(c_int), parameter, public :: AMDK_OK = 0, AMDK_ERR_NOT_PD = 4
-  integer(c_int), parameter, public :: AMDK_PREC_FP64 = 0, AMDK_PREC_MIXED = 1
+  integer(c_int), parameter, public :: AMDK_PREC_FP64 = 0, AMDK_PREC_MIXED = 1, AMDK_PREC_BF16 = 2
   integer(c_int), parameter, public :: AMDK_XC_TETER93 = 0, AMDK_XC_PZ = 1, AMDK_XC_PBE = 2
 
   type, bind(C), public :: amdk_desc

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libamdk.so")
 
 OK, ERR_ARG, ERR_DEVICE, ERR_OOM, ERR_NOT_PD, ERR_NONFINITE, ERR_STATE = range(7)
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_BF16 = 0, 1, 2
 
 # Exported symbols with (restype, argtypes).  Kept in one table so the CPU
 # test can check that every declaration in include/amdk.h is exported.
@@ -96,6 +96,7 @@ SIGNATURES = {
     "amdk_ekin": (_i, [_vp, _dp, _i, _dp, _i, _dp]),
     "amdk_enl": (_i, [_vp, _dp, _dp, _i, _dp]),
     "amdk_ion_energies": (_i, [_vp, _dp, _dp]),
+    "amdk_set_ortho_method": (_i, [_vp, _i]),
     "amdk_profile_enable": (_i, [_vp, _i]),
     "amdk_profile_names": (_i, [_vp, C.c_char_p, _i]),
     "amdk_profile_query": (_i, [_vp, C.c_char_p, _dp, _ip]),

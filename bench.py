@@ -89,7 +89,7 @@ def kernel_bytes(ctx, pw, nb, prec):
     }
 
 
-def run_gpu(sysobj, prec, steps, warmup, c0, profile_steps=3, tfor_steps=0):
+def run_gpu(sysobj, prec, steps, warmup, c0, profile_steps=3, tfor_steps=0, ortho="cholesky"):
     import torch
 
     from amdkiit_amd.context import KSContext
@@ -98,6 +98,7 @@ def run_gpu(sysobj, prec, steps, warmup, c0, profile_steps=3, tfor_steps=0):
     ctx = KSContext(s.pw, s.nbands, prec)
     ctx.set_species(s.tables)
     ctx.set_positions(s.sp, s.tau)
+    ctx.set_ortho_method(ortho)
     c = ctx.ortho(c0)
     ctx.wfn_load(c, s.occ)
     it = 0
@@ -221,8 +222,9 @@ def cpu_baseline(s, c0, seconds_hint, cores=None):
             "s_per_step": dt, "warmup_s": t1 - t0}
 
 
-def measure_secondary(cfg_name, sysobj, steps, warmup, seed):
-    """Mixed and FP64 steps/s of another workload from one seeded start."""
+def measure_secondary(cfg_name, sysobj, steps, warmup, seed, bf16=False):
+    """Mixed and FP64 steps/s of another workload from one seeded start (and
+    with bf16, the BF16 recipe with FP64 Loewdin orthonormalisation)."""
     s2 = sysobj
     rng2 = np.random.default_rng(seed)
     c02 = (rng2.random((s2.nbands, s2.pw.ngpw)) - 0.5 + 1j * (rng2.random((s2.nbands, s2.pw.ngpw)) - 0.5)) / (1.0 + s2.pw.g2_pw)
@@ -234,7 +236,18 @@ def measure_secondary(cfg_name, sysobj, steps, warmup, seed):
     cx.close()
     c64, d64, e64b, p64, _ = run_gpu(s2, "fp64", steps, warmup, c02, profile_steps=1)
     c64.close()
+    rb = None
+    if bf16:
+        cb, db, eb, pb, _ = run_gpu(s2, "bf16", steps, warmup, c02, profile_steps=1, ortho="lowdin")
+        cb.close()
+        rb = {"recipe": "BF16 projector/Lagrange products with one refinement term, FP64 Loewdin",
+              "steps_per_s": steps / db, "ms_per_step": db / steps * 1e3, "speedup_vs_fp64": d64 / db,
+              "etot_diff_per_atom_vs_fp64": abs(eb["etot"] - e64b["etot"]) / s2.nat,
+              "ms_by_region": {k: v[0] for k, v in sorted(pb.items(), key=lambda x: -x[1][0])},
+              "note": "same seeded start and step count; the Loewdin path differs from the Cholesky one "
+                      "step by step and meets it at convergence"}
     return {"workload": cfg_name, "steps": steps, "grid": list(s2.pw.n), "ngpw": s2.pw.ngpw, "bands": s2.nbands,
+            "bf16_lowdin": rb,
             "mixed_steps_per_s": steps / d2, "mixed_ms_per_step": d2 / steps * 1e3,
             "fp64_steps_per_s": steps / d64, "fp64_ms_per_step": d64 / steps * 1e3, "mixed_speedup": d64 / d2,
             "hpsi_ms": hp2, "hpsi_GBps": hb2 / (hp2 * 1e-3) / 1e9,
@@ -306,7 +319,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="h2o_32_pbe", choices=sorted(CONFIGS))
-    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64", "bf16"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="skip the 64 H2O secondary measurement")
@@ -388,7 +401,7 @@ def main():
     if not a.no_secondary and a.config == "h2o_32_pbe":
         log("[bench] secondary: 64 H2O")
         secondary = measure_secondary(CONFIGS["h2o_64_pbe"], systems.CONFIGS["h2o_64_pbe"](),
-                                      max(5, a.steps // 2), a.warmup, 2025)
+                                      max(5, a.steps // 2), a.warmup, 2025, bf16=True)
         log(f"[bench] 64 H2O: mixed {secondary['mixed_ms_per_step']:.2f} ms/step, fp64 speedup "
             f"{secondary['mixed_speedup']:.2f}x, H|psi> {secondary['hpsi_GBps']:.0f} GB/s")
         log("[bench] secondary: 128 H2O")
@@ -440,7 +453,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if a.precision == "mixed" else "f64",
+        "dtype": {"mixed": "f32", "fp64": "f64", "bf16": "bf16+f32"}[a.precision],
         "data": "synthetic (seeded water lattice + seeded random coefficients)",
         "config": {"workload": CONFIGS[a.config], "precision": a.precision, "grid": list(s.pw.n),
                    "ngpw": s.pw.ngpw, "bands": s.nbands, "atoms": s.nat, "xc": s.xc},

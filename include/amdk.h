@@ -47,7 +47,11 @@ enum {
   AMDK_ERR_STATE = 6       /* required table (V_KS, projectors, positions) not set */
 };
 
-enum { AMDK_PREC_FP64 = 0, AMDK_PREC_MIXED = 1 };
+/* AMDK_PREC_BF16: AMDK_PREC_MIXED with the projector and Lagrange-multiplier
+ * products in BF16 with one refinement term: operands split into hi =
+ * bf16(x) and lo = bf16(x - hi), products hi*hi' + hi*lo' + lo*hi' on the
+ * BF16 matrix cores, FP32 sums over 32-deep tiles, FP64 across tiles. */
+enum { AMDK_PREC_FP64 = 0, AMDK_PREC_MIXED = 1, AMDK_PREC_BF16 = 2 };
 
 typedef struct amdk_ctx amdk_ctx;
 
@@ -247,6 +251,16 @@ int amdk_pcg(amdk_ctx* ctx, double* c0, const double* c2, double* hnm1, int ldc,
 int amdk_ekin(amdk_ctx* ctx, const double* c0, int ldc, const double* occ, int nb, double* ekin);
 int amdk_enl(amdk_ctx* ctx, const double* nl, const double* occ, int nb, double* enl);
 int amdk_ion_energies(amdk_ctx* ctx, double* esr, double* eself);
+
+/* Orthonormalisation inside amdk_scf_step / amdk_scf_step_tfor:
+ * AMDK_ORTHO_CHOLESKY (default) is ortho_gs (wfn_initialize.f90:448-558);
+ * AMDK_ORTHO_LOWDIN is C <- C S^-1/2 in FP64 (S^-1/2 by a coupled
+ * Newton-Schulz iteration, for bands that were orthonormal before the step's
+ * PCG update).  No counterpart in the reference, which uses Cholesky only;
+ * the converged state is the same, the SCF path differs.  amdk_ortho and
+ * amdk_wfn_ortho stay Cholesky. */
+enum { AMDK_ORTHO_CHOLESKY = 0, AMDK_ORTHO_LOWDIN = 1 };
+int amdk_set_ortho_method(amdk_ctx* ctx, int method);
 
 /* Per-kernel timing with HIP events on the context stream. */
 int amdk_profile_enable(amdk_ctx* ctx, int on);
