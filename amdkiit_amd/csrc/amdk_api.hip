@@ -294,13 +294,18 @@ static bool reg_ok(const amdk_ctx* c, int n) {
   }
 }
 
-// Launch shapes of the register passes: TX x columns per block (y and z
-// passes; 16 measured faster than 32 for FP32 100-point columns: more,
-// smaller workgroups per CU), HR half rows per block (x passes), PG band
-// pairs sharing one V(r) tile in the z pass of H|psi> (2, 4 and 8 measured
-// equal within 1 %).
+// Launch shapes of the register passes: TX x columns per block (y passes,
+// x tiles within one plane row: walking the columns across planes instead
+// measured slower, these passes being bound by memory), ZTX columns per
+// block for the z passes, which walk the n1 x n2 columns in one x-fastest
+// sequence, so no x tile is partly past the row end: 16, or
+// 32 where 16 * TM threads would leave the last wave partly empty and 32 * TM
+// fills whole waves (TM = 10: 320 threads, five full waves, instead of 160).
+// HR half rows per block (x passes), PG band pairs sharing one V(r) tile in
+// the z pass of H|psi> (2, 4 and 8 measured equal within 1 %).
 template <typename T, int TM> struct RegTiles {
   static constexpr int TX = 16;
+  static constexpr int ZTX = (16 * TM) % 64 != 0 && (32 * TM) % 64 == 0 ? 32 : 16;
   static constexpr int HR = 240 / (2 * TM);
   static constexpr int PG = sizeof(T) == 4 ? 4 : 2;
 };
@@ -308,9 +313,9 @@ template <typename T, int TM> struct RegTiles {
 template <typename T, int A, int B>
 static void reg_z_vpsi(amdk_ctx* c, const Sphere& S, int np) {
   using V = typename cplx_of<T>::type;
-  constexpr int N = A * B, TM = MaxAB<A, B>::v, TX = RegTiles<T, TM>::TX, PG = RegTiles<T, TM>::PG;
+  constexpr int N = A * B, TM = MaxAB<A, B>::v, TX = RegTiles<T, TM>::ZTX, PG = RegTiles<T, TM>::PG;
   const size_t lds = (size_t)(N + N * TX) * sizeof(V);
-  dim3 grid((c->n1 + TX - 1) / TX, c->n2, (np + PG - 1) / PG);
+  dim3 grid((c->n1 * c->n2 + TX - 1) / TX, 1, (np + PG - 1) / PG);
   k_z_vpsi_r<T, A, B, TX><<<grid, TX * TM, lds, c->stream>>>(geom_of(c, S), (V*)c->W2, (const T*)c->vr, np, PG,
                                                             (const V*)(sizeof(T) == 4 ? c->tw_f[2] : c->tw_d[2]));
 }
@@ -318,9 +323,9 @@ template <typename T, int A, int B>
 static void reg_z_rho(amdk_ctx* c, const Sphere& S, int np, double* rho, int accumulate, const double* wpair,
                       int groups, size_t part_stride) {
   using V = typename cplx_of<T>::type;
-  constexpr int N = A * B, TM = MaxAB<A, B>::v, TX = RegTiles<T, TM>::TX;
+  constexpr int N = A * B, TM = MaxAB<A, B>::v, TX = RegTiles<T, TM>::ZTX;
   const size_t lds = (size_t)(N + N * TX) * sizeof(V);
-  dim3 grid((c->n1 + TX - 1) / TX, c->n2, groups);
+  dim3 grid((c->n1 * c->n2 + TX - 1) / TX, 1, groups);
   k_z_rho_r<T, A, B, TX><<<grid, TX * TM, lds, c->stream>>>(geom_of(c, S), (const V*)c->W2, rho, np, wpair, accumulate,
                                                            (const V*)(sizeof(T) == 4 ? c->tw_f[2] : c->tw_d[2]),
                                                            part_stride);

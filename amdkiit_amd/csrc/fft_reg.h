@@ -130,9 +130,10 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
   V* lds = tw + N;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int n1 = g.n1, n2 = g.n2;
-  const int x = blockIdx.x * TX + c;
-  const int y = blockIdx.y;
-  const bool live = x < n1;
+  // columns of the n1 x n2 plane in x-fastest order, TX consecutive per block
+  const int q = blockIdx.x * TX + c;
+  const bool live = q < n1 * n2;
+  const int y = live ? q / n1 : 0, x = q - y * n1;
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twg[i];
   T vv[A];
 #pragma unroll
@@ -191,9 +192,9 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
   V* lds = tw + N;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int n1 = g.n1, n2 = g.n2;
-  const int x = blockIdx.x * TX + c;
-  const int y = blockIdx.y;
-  const bool live = x < n1;
+  const int q = blockIdx.x * TX + c;  // plane column, x fastest (as in k_z_vpsi_r)
+  const bool live = q < n1 * n2;
+  const int y = live ? q / n1 : 0, x = q - y * n1;
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twg[i];
   __syncthreads();
   double acc[A];
