@@ -106,6 +106,9 @@ class KSContext:
         d.nproj, d.proj_l = L.iptr(k["nproj"]), L.iptr(k["proj_l"])
         d.twnl, d.dmat = L.dptr(k["twnl"]), L.dptr(k["dmat"])
         self._chk(self.lib.amdk_set_species(self.h, C.byref(d)))
+        # hess's WSG weights (zero for UPF species, as in the reference)
+        k["wsg"] = np.ascontiguousarray(T.wsg, np.float64)
+        self._chk(self.lib.amdk_set_precon_weights(self.h, L.dptr(k["wsg"])))
 
     def set_positions(self, sp_of_atom, tau):
         """exp_igrxfactor + E_SR for positions tau (nat, 3) bohr."""

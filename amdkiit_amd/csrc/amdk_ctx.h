@@ -80,6 +80,8 @@ struct amdk_ctx {
   int nsp = 0, maxproj = 0;
   std::vector<int> h_nproj, h_proj_l;
   std::vector<double> h_dmat, h_zv;
+  std::vector<double> h_wsg;  // [nsp][maxproj] PCG preconditioner weights (WSG of hess)
+  int maxproj_caller = 0;     // maxproj of the caller's species descriptor (row length of its tables)
   double* d_vps = nullptr;    // [nsp][ldc_rho]
   double* d_rhos = nullptr;   // [nsp][ldc_rho]
   double* d_twnl = nullptr;   // [nsp][maxproj][ldc_pw]

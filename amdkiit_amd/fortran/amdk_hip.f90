@@ -38,7 +38,7 @@ module amdk_hip
   end type
 
   public :: amdk_ctx_create, amdk_ctx_destroy, amdk_set_vks, amdk_vpsi, amdk_density, &
-            amdk_set_species, amdk_set_positions, amdk_nproj, amdk_fnl, amdk_fnonloc, &
+            amdk_set_species, amdk_set_precon_weights, amdk_set_positions, amdk_nproj, amdk_fnl, amdk_fnonloc, &
             amdk_overlap, amdk_lagrange, amdk_ortho, amdk_vks, amdk_forces, &
             amdk_wfn_load, amdk_wfn_store, amdk_scf_step, amdk_scf_forces, amdk_scf_step_tfor, &
             amdk_wfn_ortho
@@ -84,6 +84,11 @@ module amdk_hip
       import :: c_int, c_ptr, amdk_species
       type(c_ptr), value :: ctx
       type(amdk_species), intent(in) :: s
+    end function
+    integer(c_int) function amdk_set_precon_weights(ctx, wsg) bind(C, name="amdk_set_precon_weights")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: wsg(*)
     end function
     integer(c_int) function amdk_set_positions(ctx, nat, sp, tau) bind(C, name="amdk_set_positions")
       import :: c_int, c_ptr, c_double

@@ -99,14 +99,25 @@ contains
   ! twnl(ngpw, nghmax, sp_t), ngh(sp_t), nghtol(nghmax, sp_t) and the
   ! projector coupling dmat(nghmax, nghmax, sp_t) (wsg on the diagonal, or
   ! hlsg).
-  subroutine amdk_ref_species(zv, vps, rhos, ngh, nghtol, twnl, dmat)
+  ! wsg: the reference's WSG(spmax, maxngh) as hess reads it (cg.f90:20-40);
+  ! zero for UPF species, whose path never sets it.  Without it the library
+  ! takes the diagonal of dmat.
+  subroutine amdk_ref_species(zv, vps, rhos, ngh, nghtol, twnl, dmat, wsg)
     real(c_double), intent(in), target :: zv(:), vps(:,:), rhos(:,:), twnl(:,:,:), dmat(:,:,:)
     integer(c_int), intent(in), target :: ngh(:), nghtol(:,:)
+    real(c_double), intent(in), optional :: wsg(:,:)
     type(amdk_species) :: s
+    real(c_double), allocatable :: w(:,:)
     s%nsp = int(sp_t, c_int); s%maxproj = int(nghmax, c_int)
     s%zv = c_loc(zv); s%vps = c_loc(vps); s%rhos = c_loc(rhos)
     s%nproj = c_loc(ngh); s%proj_l = c_loc(nghtol); s%twnl = c_loc(twnl); s%dmat = c_loc(dmat)
     call amdk_check(amdk_context, amdk_set_species(amdk_context, s), 'amdk_ref_species')
+    if (present(wsg)) then
+      allocate(w(nghmax, sp_t))          ! projector fastest, as the library reads it
+      w = transpose(wsg(1:sp_t, 1:nghmax))
+      call amdk_check(amdk_context, amdk_set_precon_weights(amdk_context, w), 'amdk_ref_species (wsg)')
+      deallocate(w)
+    end if
   end subroutine
 
   subroutine amdk_ref_finalize()

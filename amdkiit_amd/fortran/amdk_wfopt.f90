@@ -9,7 +9,7 @@
 !   a_cell(3,3)  mill_pw(3,ngpw)  mill_rho(3,ngrho)  occupation(nstate)
 !   atom_p_sp(nsp)  atco(3,na_max,nsp)
 !   zv(nsp)  vps(ngrho,nsp)  rhos(ngrho,nsp)  ngh(nsp)  nghtol(nghmax,nsp)
-!   twnl(ngpw,nghmax,nsp)  dmat(nghmax,nghmax,nsp)  c_0(ngpw,nstate) complex
+!   twnl(ngpw,nghmax,nsp)  dmat(nghmax,nghmax,nsp)  wsg(nsp,nghmax)  c_0(ngpw,nstate) complex
 ! Output (stream): per step etotal eke eht epseu enl exc esr eself gemax cnorm,
 !   then force(3,na_max,nsp), then c_0(ngpw,nstate).
 program amdk_wfopt
@@ -22,6 +22,7 @@ program amdk_wfopt
   real(c_double) :: a(3, 3)
   integer(c_int), allocatable, target :: mpw(:,:), mrho(:,:), ngh(:), nghtol(:,:), aps(:)
   real(c_double), allocatable, target :: occ(:), xat(:,:,:), zv(:), vps(:,:), rhos(:,:), twnl(:,:,:), dmat(:,:,:)
+  real(c_double), allocatable :: wsg(:,:)
   complex(c_double_complex), allocatable :: c_in(:,:)
   character(len=512) :: fin, fout
 
@@ -33,12 +34,12 @@ program amdk_wfopt
   nghm = hdr(10); ngpw = hdr(11); ngrho = hdr(12); nstep = hdr(13)
   allocate(mpw(3, ngpw), mrho(3, ngrho), occ(nst), aps(nsp), xat(3, namax, nsp))
   allocate(zv(nsp), vps(ngrho, nsp), rhos(ngrho, nsp), ngh(nsp), nghtol(nghm, nsp))
-  allocate(twnl(ngpw, nghm, nsp), dmat(nghm, nghm, nsp), c_in(ngpw, nst))
-  read(u) a, mpw, mrho, occ, aps, xat, zv, vps, rhos, ngh, nghtol, twnl, dmat, c_in
+  allocate(twnl(ngpw, nghm, nsp), dmat(nghm, nghm, nsp), wsg(nsp, nghm), c_in(ngpw, nst))
+  read(u) a, mpw, mrho, occ, aps, xat, zv, vps, rhos, ngh, nghtol, twnl, dmat, wsg, c_in
   close(u)
 
   call amdk_ref_init(n, a, symm, ngpw, mpw, ngrho, mrho, nst, prec, xc, nsp, aps, namax, nghm)
-  call amdk_ref_species(zv, vps, rhos, ngh, nghtol, twnl, dmat)
+  call amdk_ref_species(zv, vps, rhos, ngh, nghtol, twnl, dmat, wsg)
   occupation = occ
   atco = xat
   call exp_igrxfactor()

@@ -70,6 +70,7 @@ SIGNATURES = {
     "amdk_density": (_i, [_vp, _dp, _i, _dp, _i, _dp]),
     "amdk_density_dev": (_i, [_vp, _vp, _dp, _i, _vp]),
     "amdk_set_species": (_i, [_vp, C.POINTER(Species)]),
+    "amdk_set_precon_weights": (_i, [_vp, _dp]),
     "amdk_set_positions": (_i, [_vp, _i, _ip, _dp]),
     "amdk_nproj": (_i, [_vp]),
     "amdk_fnl": (_i, [_vp, _dp, _i, _i, _dp]),

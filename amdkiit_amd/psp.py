@@ -155,6 +155,7 @@ class Species:
     local(g2)      -> V_s(G) * Omega (short-range local part, see module doc)
     projectors     -> list of (l, lm_index, radial(g2) -> 4 pi int r^2 p j_l)
     D              -> coupling matrix over the projector list
+    precon_weights -> WSG(is, iv) as the PCG preconditioner reads it
     """
 
     zv = 0.0
@@ -168,6 +169,12 @@ class Species:
 
     def dmat(self):
         return np.zeros((0, 0))
+
+    def precon_weights(self):
+        """Per-projector weights of the nonlocal term of hess (cg.f90:20-40):
+        WSG(is, iv), the Kleinman-Bylander weights of the CPMD format, i.e. the
+        diagonal of D."""
+        return np.diag(self.dmat()).copy()
 
 
 def read_cpmd_psp(path, lmax_read):
@@ -485,6 +492,13 @@ class UpfSpecies(Species):
     def dmat(self):
         return self._D
 
+    def precon_weights(self):
+        """Zero: hess (cg.f90:20-28) reads WSG(is, iv) for UPF species too,
+        but only the CPMD-format reader sets WSG (pseudopotential.f90:480-489),
+        so for a UPF species the preconditioner has no nonlocal term (WSG is a
+        static module array, system_data_types.f90:78)."""
+        return np.zeros(len(self._proj))
+
 
 def _sph_jl_vec(l, x):
     x = np.asarray(x, np.float64)
@@ -550,4 +564,9 @@ class Tables:
                 self.proj_l[isp, k] = l
             D = s.dmat()
             self.dmat[isp, :D.shape[0], :D.shape[1]] = D
+        # PCG preconditioner weights WSG(is, iv) (hess, cg.f90:20-40)
+        self.wsg = np.zeros((self.nsp, self.maxproj))
+        for isp, s in enumerate(species):
+            w = s.precon_weights()
+            self.wsg[isp, :len(w)] = w
 

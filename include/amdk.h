@@ -115,6 +115,15 @@ typedef struct {
 } amdk_species;
 int amdk_set_species(amdk_ctx* ctx, const amdk_species* s);
 
+/* PCG preconditioner weights WSG(is, iv) of hess (cg.f90:20-40), which adds
+ * N_s WSG(s, k) twnl_k(G)^2 per projector to K(G): wsg[s * maxproj + k], with
+ * the maxproj of the species descriptor.  amdk_set_species sets them to the
+ * diagonal of dmat (the KB weights of the CPMD format); the reference never
+ * sets WSG for UPF species (it is filled only by nlppgrid,
+ * pseudopotential.f90:480-489), so its callers pass zeros for them.  Takes
+ * effect at once if positions are set, else at amdk_set_positions. */
+int amdk_set_precon_weights(amdk_ctx* ctx, const double* wsg);
+
 /* Atomic positions (bohr, tau[3*a+k]) and species of each atom.  Builds the
  * structure factors, sum_a V_s e^{-iGR}, sum_a rho_s e^{-iGR}, the projector
  * table beta = (-i)^l twnl e^{-iGR} and the short-range ion energy/forces.

@@ -261,12 +261,14 @@ class KSOracle:
         return c2, gemax, cnorm
 
     def hess(self):
-        """Diagonal preconditioner K(G)^-1 (hess, cg.f90:3-61), HCUT = 0.5."""
+        """Diagonal preconditioner K(G)^-1 (hess, cg.f90:3-61), HCUT = 0.5;
+        the nonlocal term weighs twnl^2 by WSG(is, iv) (cg.f90:26,38), which is
+        zero for UPF species (only the CPMD-format reader sets it)."""
         vpp = np.full(self.pw.ngpw, float(np.real(self.eigrxvps[0])))
         natsp = np.bincount(self.sp, minlength=self.T.nsp)
         for s in range(self.T.nsp):
             for k in range(self.T.nproj[s]):
-                vpp += natsp[s] * self.T.dmat[s, k, k] * self.T.twnl[s, k] ** 2
+                vpp += natsp[s] * self.T.wsg[s, k] * self.T.twnl[s, k] ** 2
         vpp = 0.5 * self.g2p + vpp
         vpp = np.maximum(vpp, 0.5)
         return 1.0 / vpp

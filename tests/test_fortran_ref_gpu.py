@@ -60,6 +60,8 @@ def write_setup(path, s, c0, prec, nstep):
         tab(np.asarray(T.twnl).reshape(nsp, -1, s.pw.ngpw), (nsp, ngh_max, s.pw.ngpw)).tofile(f)
         tab(np.asarray(T.dmat).reshape(nsp, T.maxproj, T.maxproj) if T.maxproj else np.zeros((nsp, 0, 0)),
             (nsp, ngh_max, ngh_max)).tofile(f)
+        # WSG(nsp, nghmax) in Fortran order: species fastest
+        np.ascontiguousarray(tab(T.wsg, (nsp, ngh_max)).T).tofile(f)
         np.ascontiguousarray(c0, np.complex128).tofile(f)
     return counts, na_max
 
