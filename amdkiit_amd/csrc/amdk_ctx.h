@@ -126,6 +126,8 @@ struct amdk_ctx {
   bool rhog_from_state = false;
   // reductions and band-space matrices
   double* d_part = nullptr;   // [8][RED_BLOCKS]
+  double* d_npart = nullptr;  // per-block band-norm partials of the Lagrange update
+  int npart_stride = 0;
   double* d_red = nullptr;    // [16]
   double* d_gpart = nullptr;  // split-K partials
   size_t gpart_cap = 0;

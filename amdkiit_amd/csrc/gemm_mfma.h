@@ -412,6 +412,136 @@ __global__ void __launch_bounds__(512) k_ovl_bf16(const double* __restrict__ X, 
 
 constexpr int UPM_MC = 32, UPM_NB = 128, UPM_K = 64;
 
+// Band norms of the updated Y, reduced by the update kernel that writes it
+// (what k_grad_norms computes in a separate pass over C2): for bands
+// n < nocc and complex elements g < npw, with DOTP weights (1 for Re at
+// G = 0, 0 for Im at G = 0, 2 otherwise),
+//   part[L]              = sum w_n DOTP(Y_n, Y_n)                 (CNORM)
+//   part[stride + L]     = max |Y_n(G)|                            (GEMAX)
+//   part[2 stride + L]   = sum w_n (finv_n vpp_g)^2 DOTP(Y_n, Y_n) (PCG ||P||^2)
+// for logical block L; k_fold_norms turns these into the step's reduction
+// slots in a fixed order.  part == nullptr: nothing is computed.
+struct UpdNorms {
+  const double* vpp;
+  const double* finv;
+  const double* w;
+  int nocc, npw;
+  double* part;
+  int stride;
+};
+
+// The block's band weights for the norms, staged in LDS before the main
+// loop (so that the epilogue reads no global memory in data-dependent
+// order): wn_s[i] = w_n, fn_s[i] = finv_n for n = n0 + i < nocc, else 0.
+__device__ __forceinline__ void upd_norms_stage(const UpdNorms& nm, int n0, double* wn_s, double* fn_s) {
+  if (nm.part == nullptr) return;
+  for (int i = threadIdx.x; i < UPM_NB; i += blockDim.x) {
+    const int n = n0 + i;
+    const bool ok = n < nm.nocc;
+    wn_s[i] = ok ? nm.w[n] : 0.0;
+    fn_s[i] = ok ? nm.finv[n] : 0.0;
+  }
+}
+
+// x of the neighbouring lane (l ^ 1): a quad permute (1, 0, 3, 2) of both
+// halves; every lane of the wave must be active.
+__device__ __forceinline__ double lane_xor1(double x) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), 0xB1, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), 0xB1, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+
+// Store Y(n, k) = beta Y + acc (0 at the imaginary part of G = 0 when
+// zero_g0_im) for the NT tiles of this lane, rows n0 + 16 t + row(lk, v),
+// and reduce the block's norm partials when asked (weights staged by
+// upd_norms_stage).  Called by every thread of the block (the norm
+// reduction synchronises it).
+template <int NT, typename Row>
+__device__ __forceinline__ void upd_store(const dbl4* acc, Row row, double* __restrict__ Y, int ldy, int N, int K,
+                                          int k, int n0, int lk, double beta, int zero_g0_im, const UpdNorms& nm,
+                                          const double* wn_s, const double* fn_s, int L) {
+  const bool kin = k < K;
+  const bool norms = nm.part != nullptr;
+  const int g = k >> 1;
+  const bool gin = kin && g < nm.npw;
+  const double wd = k == 0 ? 1.0 : (k == 1 ? 0.0 : 2.0);
+  const double vg = norms ? nm.vpp[gin ? g : 0] : 0.0;
+  double s5 = 0.0, s7 = 0.0, m2 = 0.0;
+  // two tiles at a time: their old Y first, unconditionally (out-of-range
+  // elements read element 0 and are not stored), so that eight loads are in
+  // flight together instead of one branch and one wait each
+  constexpr int TB = 2;
+  static_assert(NT % TB == 0, "tiles in pairs");
+#pragma unroll
+  for (int t0 = 0; t0 < NT; t0 += TB) {
+    double yo[TB][4];
+    if (beta != 0.0) {
+#pragma unroll
+      for (int u = 0; u < TB; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int n = n0 + 16 * (t0 + u) + row(lk, v);
+          yo[u][v] = Y[(kin && n < N) ? (size_t)n * ldy + k : 0];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < TB; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int t = t0 + u;
+      const int n = n0 + 16 * t + row(lk, v);
+      const double val = (zero_g0_im && k == 1) ? 0.0 : (beta != 0.0 ? beta * yo[u][v] : 0.0) + acc[t][v];
+      if (kin && n < N) Y[(size_t)n * ldy + k] = val;
+      if (norms) {
+        const double y2 = (gin && n < nm.nocc) ? val * val : 0.0;
+        m2 = fmax(m2, y2 + lane_xor1(y2));  // |Y_n(G)|^2: the other component is in lane l ^ 1
+        const double wn = wn_s[n - n0], f = fn_s[n - n0] * vg;
+        s5 += wn * wd * y2;
+        s7 += wn * f * f * wd * y2;
+      }
+    }
+  }
+  if (!norms) return;  // uniform over the block
+  __shared__ double red[3][16];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    s5 += __shfl_down(s5, o, 64);
+    s7 += __shfl_down(s7, o, 64);
+    m2 = fmax(m2, __shfl_down(m2, o, 64));
+  }
+  const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = s5;
+    red[1][w] = s7;
+    red[2][w] = m2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int i = 0; i < nw; ++i) {
+      a += red[0][i];
+      b += red[1][i];
+      c = fmax(c, red[2][i]);
+    }
+    nm.part[L] = a;
+    nm.part[nm.stride + L] = sqrt(c);
+    nm.part[2 * nm.stride + L] = b;
+  }
+}
+
+// Fold the n block partials of upd_store into RED_BLOCKS partials per slot
+// (out + q * nred for q = 0 CNORM, 1 GEMAX (max), 2 ||P||^2), each a fixed
+// strided subset in a fixed order.
+__global__ void k_fold_norms(const double* __restrict__ part, int stride, int n, double* __restrict__ out, int nred) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * nred) return;
+  const int q = i / nred, r = i - q * nred;
+  const double* p = part + (size_t)q * stride;
+  double v = 0.0;
+  for (int j = r; j < n; j += nred) v = q == 1 ? fmax(v, p[j]) : v + p[j];
+  out[(size_t)q * nred + r] = v;
+}
+
 // Block: 64 reals of k (4 waves x 16) x 128 columns n (8 MFMA tiles per
 // wave); logical block = (k-block, n-block), n fastest, so the n-blocks that
 // read the same rows of X share an XCD's L2.  Chunks of 32 m: X (32 x 64)
@@ -430,7 +560,7 @@ __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X1,
                                                   const double* __restrict__ X2, int M2,
                                                   const double* __restrict__ Q2, int ldq2, double a2,
                                                   double* __restrict__ Y, int ldy, int N, int K,
-                                                  double beta, int per, int zero_g0_im) {
+                                                  double beta, int per, int zero_g0_im, UpdNorms nm) {
   typedef typename Acc4<TC>::type AccT;
   constexpr bool F64 = sizeof(TC) == 8;
   constexpr int XL = (UPM_MC * UPM_K / 2) / 256;   // double2 of X per thread
@@ -444,6 +574,8 @@ __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X1,
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = lane & 15, lk = lane >> 4;
   const int kl = wave * 16 + li, k = k0 + kl;
+  __shared__ double wn_s[UPM_NB], fn_s[UPM_NB];
+  upd_norms_stage(nm, n0, wn_s, fn_s);  // visible after the main loop's first barrier
   dbl4 acc[8];
   AccT tac[8];
 #pragma unroll
@@ -513,17 +645,22 @@ __global__ void __launch_bounds__(256) k_upd_mfma(const double* __restrict__ X1,
     }
     __syncthreads();
   }
-  if (k < K) {
+  if constexpr (TRI) {  // C U^-1 (no norms): the plain store
+    if (k < K) {
 #pragma unroll
-    for (int t = 0; t < 8; ++t)
+      for (int t = 0; t < 8; ++t)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int n = n0 + 16 * t + acc_row<TC>(lk, v);
-        if (n < N) {
-          double* y = Y + (size_t)n * ldy + k;
-          *y = (zero_g0_im && k == 1) ? 0.0 : (beta != 0.0 ? beta * *y : 0.0) + acc[t][v];
+        for (int v = 0; v < 4; ++v) {
+          const int n = n0 + 16 * t + acc_row<TC>(lk, v);
+          if (n < N) {
+            double* y = Y + (size_t)n * ldy + k;
+            *y = (zero_g0_im && k == 1) ? 0.0 : (beta != 0.0 ? beta * *y : 0.0) + acc[t][v];
+          }
         }
-      }
+    }
+  } else {
+    upd_store<8>(acc, [](int lk, int v) { return acc_row<TC>(lk, v); }, Y, ldy, N, K, k, n0, lk, beta, zero_g0_im, nm,
+                 wn_s, fn_s, L);
   }
 }
 
@@ -539,7 +676,7 @@ __global__ void __launch_bounds__(256) k_upd_bf16(const double* __restrict__ X1,
                                                   const double* __restrict__ X2, int M2,
                                                   const double* __restrict__ Q2, int ldq2, double a2,
                                                   double* __restrict__ Y, int ldy, int N, int K,
-                                                  double beta, int per, int zero_g0_im) {
+                                                  double beta, int per, int zero_g0_im, UpdNorms nm) {
   constexpr int XL = (UPM_MC * UPM_K / 2) / 256;
   constexpr int QL = (UPM_MC * UPM_NB / 2) / 256;
   __shared__ __attribute__((aligned(16))) __bf16 qh[UPM_NB][UPH_P];
@@ -553,6 +690,8 @@ __global__ void __launch_bounds__(256) k_upd_bf16(const double* __restrict__ X1,
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = lane & 15, lk = lane >> 4;
   const int kl = wave * 16 + li, k = k0 + kl;
+  __shared__ double wn_s[UPM_NB], fn_s[UPM_NB];
+  upd_norms_stage(nm, n0, wn_s, fn_s);  // visible after the main loop's first barrier
   dbl4 acc[8];
   flt4 tac[8];
 #pragma unroll
@@ -615,18 +754,8 @@ __global__ void __launch_bounds__(256) k_upd_bf16(const double* __restrict__ X1,
     for (int t = 0; t < 8; ++t) acc_flush(acc[t], tac[t]);
     __syncthreads();
   }
-  if (k < K) {
-#pragma unroll
-    for (int t = 0; t < 8; ++t)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int n = n0 + 16 * t + 4 * lk + v;
-        if (n < N) {
-          double* y = Y + (size_t)n * ldy + k;
-          *y = (zero_g0_im && k == 1) ? 0.0 : (beta != 0.0 ? beta * *y : 0.0) + acc[t][v];
-        }
-      }
-  }
+  upd_store<8>(acc, [](int lk, int v) { return 4 * lk + v; }, Y, ldy, N, K, k, n0, lk, beta, zero_g0_im, nm, wn_s,
+               fn_s, L);
 }
 
 // value of v in lane l (l uniform across the wave)
