@@ -315,9 +315,10 @@ static void reg_z_vpsi(amdk_ctx* c, const Sphere& S, int np) {
   using V = typename cplx_of<T>::type;
   constexpr int N = A * B, TM = MaxAB<A, B>::v, TX = RegTiles<T, TM>::ZTX, PG = RegTiles<T, TM>::PG;
   const size_t lds = (size_t)(N + N * TX) * sizeof(V);
-  dim3 grid((c->n1 * c->n2 + TX - 1) / TX, 1, (np + PG - 1) / PG);
-  k_z_vpsi_r<T, A, B, TX><<<grid, TX * TM, lds, c->stream>>>(geom_of(c, S), (V*)c->W2, (const T*)c->vr, np, PG,
-                                                            (const V*)(sizeof(T) == 4 ? c->tw_f[2] : c->tw_d[2]));
+  const int ncb = (c->n1 * c->n2 + TX - 1) / TX, ngrp = (np + PG - 1) / PG, per = (ncb * ngrp + 7) / 8;
+  k_z_vpsi_r<T, A, B, TX><<<8 * per, TX * TM, lds, c->stream>>>(geom_of(c, S), (V*)c->W2, (const T*)c->vr, np, PG,
+                                                               (const V*)(sizeof(T) == 4 ? c->tw_f[2] : c->tw_d[2]),
+                                                               ngrp, per);
 }
 template <typename T, int A, int B>
 static void reg_z_rho(amdk_ctx* c, const Sphere& S, int np, double* rho, int accumulate, const double* wpair,
@@ -325,10 +326,10 @@ static void reg_z_rho(amdk_ctx* c, const Sphere& S, int np, double* rho, int acc
   using V = typename cplx_of<T>::type;
   constexpr int N = A * B, TM = MaxAB<A, B>::v, TX = RegTiles<T, TM>::ZTX;
   const size_t lds = (size_t)(N + N * TX) * sizeof(V);
-  dim3 grid((c->n1 * c->n2 + TX - 1) / TX, 1, groups);
-  k_z_rho_r<T, A, B, TX><<<grid, TX * TM, lds, c->stream>>>(geom_of(c, S), (const V*)c->W2, rho, np, wpair, accumulate,
-                                                           (const V*)(sizeof(T) == 4 ? c->tw_f[2] : c->tw_d[2]),
-                                                           part_stride);
+  const int ncb = (c->n1 * c->n2 + TX - 1) / TX, per = (ncb * groups + 7) / 8;
+  k_z_rho_r<T, A, B, TX><<<8 * per, TX * TM, lds, c->stream>>>(geom_of(c, S), (const V*)c->W2, rho, np, wpair, accumulate,
+                                                              (const V*)(sizeof(T) == 4 ? c->tw_f[2] : c->tw_d[2]),
+                                                              part_stride, groups, per);
 }
 template <typename T, int A, int B>
 static void reg_cols(amdk_ctx* c, const Sphere& S, int np, bool inverse) {

@@ -117,12 +117,26 @@ __device__ __forceinline__ int pencil_off(int t, int k, int lo, int hi, int base
   return (f >= lo && f <= hi) ? base + (A * k) * ps - (wrap ? N * ps : 0) : -1;
 }
 
+// Block order of the z passes: MI355X hands consecutive workgroups
+// round-robin to its 8 XCDs; physical block p runs logical block
+// (p % 8) * per + p / 8, so each XCD walks one contiguous range of logical
+// blocks L = cb * ngrp + grp (launch 8 * per blocks; ids past the last
+// logical block return).  Returns false for those.
+__device__ __forceinline__ bool z_block_of(int ngrp, int per, int ncb, int& cb, int& grp) {
+  const int p = blockIdx.x;
+  const int L = (p & 7) * per + (p >> 3);
+  if (L >= ncb * ngrp) return false;
+  cb = L / ngrp;
+  grp = L - cb * ngrp;
+  return true;
+}
+
 // ---------------------------------------------------------------------------
 // z pass of H|psi>: planes |m3| <= R -> z-FFT exp(+i) -> * v(r) -> z-FFT exp(-i)
 template <typename T, int A, int B, int TX>
 __global__ void __launch_bounds__(TX * (A > B ? A : B))
     k_z_vpsi_r(PassGeom g, typename cplx_of<T>::type* __restrict__ W2, const T* __restrict__ vr, int npairs,
-               int PG, const typename cplx_of<T>::type* __restrict__ twg) {
+               int PG, const typename cplx_of<T>::type* __restrict__ twg, int ngrp, int per) {
   using V = typename cplx_of<T>::type;
   constexpr int N = A * B, TM = MaxAB<A, B>::v;
   extern __shared__ unsigned char smem_raw[];
@@ -130,8 +144,14 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
   V* lds = tw + N;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int n1 = g.n1, n2 = g.n2;
+  // logical block (column block, pair group), group fastest, walked
+  // XCD-contiguously (z_block_of): a V(r) tile's pair groups and the
+  // neighbouring column blocks, which share the 128-byte lines at their
+  // edges, run on one XCD and share its L2
+  int cb, grp;
+  if (!z_block_of(ngrp, per, (n1 * n2 + TX - 1) / TX, cb, grp)) return;
   // columns of the n1 x n2 plane in x-fastest order, TX consecutive per block
-  const int q = blockIdx.x * TX + c;
+  const int q = cb * TX + c;
   const bool live = q < n1 * n2;
   const int y = live ? q / n1 : 0, x = q - y * n1;
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twg[i];
@@ -142,7 +162,7 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
     vv[j1] = (live && t < B) ? vr[((size_t)z * n2 + y) * n1 + x] : (T)0;
   }
   __syncthreads();
-  const int pg0 = blockIdx.z * PG, pg1 = min(npairs, pg0 + PG);
+  const int pg0 = grp * PG, pg1 = min(npairs, pg0 + PG);
   const int ps = n2 * g.p1, base = (t + g.R) * ps;  // plane stride; plane kz sits at kz + R
   const bool act = t < A && live;
   for (int pl = pg0; pl < pg1; ++pl) {
@@ -184,7 +204,7 @@ template <typename T, int A, int B, int TX>
 __global__ void __launch_bounds__(TX * (A > B ? A : B))
     k_z_rho_r(PassGeom g, const typename cplx_of<T>::type* __restrict__ W2, double* __restrict__ rho, int npairs,
               const double* __restrict__ wpair, int accumulate, const typename cplx_of<T>::type* __restrict__ twg,
-              size_t part_stride) {
+              size_t part_stride, int ngrp, int per) {
   using V = typename cplx_of<T>::type;
   constexpr int N = A * B, TM = MaxAB<A, B>::v;
   extern __shared__ unsigned char smem_raw[];
@@ -192,7 +212,9 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
   V* lds = tw + N;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int n1 = g.n1, n2 = g.n2;
-  const int q = blockIdx.x * TX + c;  // plane column, x fastest (as in k_z_vpsi_r)
+  int cb, grp;  // (column block, pair group), XCD-contiguous as in k_z_vpsi_r
+  if (!z_block_of(ngrp, per, (n1 * n2 + TX - 1) / TX, cb, grp)) return;
+  const int q = cb * TX + c;  // plane column, x fastest (as in k_z_vpsi_r)
   const bool live = q < n1 * n2;
   const int y = live ? q / n1 : 0, x = q - y * n1;
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twg[i];
@@ -215,9 +237,9 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
       else { dst[k2].x = 0; dst[k2].y = 0; }
     }
   };
-  const int per = (npairs + gridDim.z - 1) / gridDim.z;
-  const int pa = blockIdx.z * per, pb = min(npairs, pa + per);
-  rho += blockIdx.z * part_stride;
+  const int ppg = (npairs + ngrp - 1) / ngrp;
+  const int pa = grp * ppg, pb = min(npairs, pa + ppg);
+  rho += grp * part_stride;
   if (PF && pa < pb) load_pair(pa, rn);
   for (int pl = pa; pl < pb; ++pl) {
     V r[TM];
