@@ -498,18 +498,22 @@ struct DenseOut {
 // z-FFT exp(-i) of full columns of a(r) + i b(r); keep planes |m3| <= R.
 template <int A, int B, int TX>
 __global__ void __launch_bounds__(TX * (A > B ? A : B))
-    k_z_fwd_dense_r(PassGeom g, DenseIn in, double2* __restrict__ W2, const double2* __restrict__ twg) {
+    k_z_fwd_dense_r(PassGeom g, DenseIn in, double2* __restrict__ W2, const double2* __restrict__ twg, int nf,
+                    int per) {
   constexpr int N = A * B, TM = MaxAB<A, B>::v;
   extern __shared__ unsigned char smem_raw[];
   double2* tw = reinterpret_cast<double2*>(smem_raw);
   double2* lds = tw + N;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int n1 = g.n1, n2 = g.n2;
-  const int x = blockIdx.x * TX + c, y = blockIdx.y, f = blockIdx.z;  // field pair f
+  int cb, f;  // (column block, field pair f), XCD-contiguous as in k_z_vpsi_r
+  if (!z_block_of(nf, per, (n1 * n2 + TX - 1) / TX, cb, f)) return;
+  const int q = cb * TX + c;  // plane column, x fastest
+  const bool live = q < n1 * n2;
+  const int y = live ? q / n1 : 0, x = q - y * n1;
   const double* __restrict__ a = in.a[f];
   const double* __restrict__ b = in.b[f];
   W2 += (size_t)f * g.nplane * g.n2 * g.p1;
-  const bool live = x < n1;
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twg[i];
   __syncthreads();
   double2 r[TM];
@@ -537,19 +541,22 @@ __global__ void __launch_bounds__(TX * (A > B ? A : B))
 template <typename T, int A, int B, int TX>
 __global__ void __launch_bounds__(TX * (A > B ? A : B))
     k_z_inv_dense_r(PassGeom g, const double2* __restrict__ W2, DenseOut<T> out, double scale,
-                    const double2* __restrict__ twg) {
+                    const double2* __restrict__ twg, int nf, int per) {
   constexpr int N = A * B, TM = MaxAB<A, B>::v;
   extern __shared__ unsigned char smem_raw[];
   double2* tw = reinterpret_cast<double2*>(smem_raw);
   double2* lds = tw + N;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int n1 = g.n1, n2 = g.n2;
-  const int x = blockIdx.x * TX + c, y = blockIdx.y, f = blockIdx.z;  // field pair f
+  int cb, f;  // (column block, field pair f), XCD-contiguous as in k_z_vpsi_r
+  if (!z_block_of(nf, per, (n1 * n2 + TX - 1) / TX, cb, f)) return;
+  const int q = cb * TX + c;  // plane column, x fastest
+  const bool live = q < n1 * n2;
+  const int y = live ? q / n1 : 0, x = q - y * n1;
   double* __restrict__ oa = out.a[f];
   double* __restrict__ ob = out.b[f];
   T* __restrict__ ov = out.v[f];
   W2 += (size_t)f * g.nplane * g.n2 * g.p1;
-  const bool live = x < n1;
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twg[i];
   __syncthreads();
   double2 r[TM];
