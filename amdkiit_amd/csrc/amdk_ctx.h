@@ -93,6 +93,9 @@ struct amdk_ctx {
   std::vector<double> h_tau;
   std::vector<int4> h_plist;  // (atom, species, k, l)
   int* d_sp = nullptr;
+  int* d_order = nullptr;     // atom indices in species order (stable)
+  int4* d_agrp = nullptr;     // atom groups: {first index into d_order, count, species, 0}
+  int nagrp = 0;
   double* d_tau = nullptr;
   double2* d_tab = nullptr;   // per-atom axis phases
   int tabW = 0, hmax = 0;

@@ -382,42 +382,72 @@ __global__ void __launch_bounds__(256) k_c0_sums(const V* __restrict__ C, int ld
 
 // Local/Hartree force POTFOR (nuclear_grad.f90:4-50):
 // F_a = 2 Omega sum_{G != 0} Re[i G e^{-iGR_a} (rho_s 4pi/G^2 conj(rho_tot) + V_s conj(rho_e))]
-// Block (a, y) sums the G of split y (interleaved 256-chunks) into
+// Block (x: atom group grp[x] of one species, y: every gridDim.y-th
+// quadruple of sphere rows, one row per wave): the group's x phase rows in
+// LDS, each row's y-z factor per atom in registers, three components per
+// atom accumulated per lane and reduced in a fixed order into
 // part[(a * gridDim.y + y) * 3 + k] (without the 2 Omega).
-__global__ void __launch_bounds__(256) k_potfor(const int* __restrict__ mill, int ng, Recip rc,
-                                                const int* __restrict__ sp, const double* __restrict__ vps,
-                                                const double* __restrict__ rhos, int ldg,
-                                                const double2* __restrict__ rhog, const double2* __restrict__ sr,
-                                                const double2* __restrict__ tab, int W, int hmax,
-                                                double* __restrict__ part) {
-  __shared__ double sh[4];
-  const int a = blockIdx.x;
-  const int s = sp[a];
-  double f0 = 0, f1 = 0, f2 = 0;
-  for (int g = 1 + blockIdx.y * blockDim.x + threadIdx.x; g < ng; g += gridDim.y * blockDim.x) {
-    double gx, gy, gz;
-    gvec_of(mill, g, rc, gx, gy, gz);
-    const double g2 = gx * gx + gy * gy + gz * gz;
-    const double2 re = rhog[g], ri = sr[g];
-    const double2 rtc = make_double2(re.x + ri.x, -(re.y + ri.y));  // conj(rho_tot)
-    const double fh = 4.0 * M_PI / g2;
-    const double rs_ = rhos[(size_t)s * ldg + g], vs_ = vps[(size_t)s * ldg + g];
-    const double2 t = make_double2(rs_ * fh * rtc.x + vs_ * re.x, rs_ * fh * rtc.y - vs_ * re.y);
-    const double2 e = eigr_of(tab, a, W, hmax, mill[3 * g], mill[3 * g + 1], mill[3 * g + 2]);
-    const double2 et = zmul(e, t);
-    // Re(i G et) = -G Im(et)
-    f0 += -gx * et.y;
-    f1 += -gy * et.y;
-    f2 += -gz * et.y;
+__global__ void __launch_bounds__(256) k_potfor(const RowInfo* __restrict__ rows, int nrow, Recip rc,
+                                                const int4* __restrict__ grp, const int* __restrict__ order,
+                                                const double* __restrict__ vps, const double* __restrict__ rhos,
+                                                int ldg, const double2* __restrict__ rhog,
+                                                const double2* __restrict__ sr, const double2* __restrict__ tab,
+                                                int W, int hmax, double* __restrict__ part) {
+  __shared__ double2 ex[PHASE_AG][PHASE_WMAX];
+  __shared__ double2 yzs[4][PHASE_AG];  // per wave: this row's y-z factors
+  __shared__ double red[4][3 * PHASE_AG];
+  const int4 gi = grp[blockIdx.x];
+  const int s = gi.z;
+  stage_x_phases(tab, W, order, gi, ex);
+  __syncthreads();
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double f[PHASE_AG][3];
+#pragma unroll
+  for (int j = 0; j < PHASE_AG; ++j) f[j][0] = f[j][1] = f[j][2] = 0.0;
+  for (int r = 4 * blockIdx.y + w; r < nrow; r += 4 * gridDim.y) {
+    const RowInfo ri = rows[r];
+    if (lane < PHASE_AG) {
+      const double2* ty = tab + (size_t)order[gi.x + min(lane, gi.y - 1)] * 3 * W + W + hmax;
+      yzs[w][lane] = zmul(ty[ri.m2], ty[W + ri.m3]);
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const double gy0 = ri.m2 * rc.g[3] + ri.m3 * rc.g[6], gy1 = ri.m2 * rc.g[4] + ri.m3 * rc.g[7],
+                 gy2 = ri.m2 * rc.g[5] + ri.m3 * rc.g[8];
+    for (int e = lane; e < ri.cnt; e += 64) {
+      const int g = ri.start + e, m1 = ri.m1_0 + e;
+      if (g == 0) continue;  // G = 0
+      const double gx = m1 * rc.g[0] + gy0, gy = m1 * rc.g[1] + gy1, gz = m1 * rc.g[2] + gy2;
+      const double g2 = gx * gx + gy * gy + gz * gz;
+      const double2 re = rhog[g], rt = sr[g];
+      const double2 rtc = make_double2(re.x + rt.x, -(re.y + rt.y));  // conj(rho_tot)
+      const double fh = 4.0 * M_PI / g2;
+      const double rs_ = rhos[(size_t)s * ldg + g], vs_ = vps[(size_t)s * ldg + g];
+      const double2 t = make_double2(rs_ * fh * rtc.x + vs_ * re.x, rs_ * fh * rtc.y - vs_ * re.y);
+#pragma unroll
+      for (int j = 0; j < PHASE_AG; ++j) {
+        const double2 et = zmul(zmul(ex[j][m1 + hmax], yzs[w][j]), t);
+        // Re(i G et) = -G Im(et)
+        f[j][0] += -gx * et.y;
+        f[j][1] += -gy * et.y;
+        f[j][2] += -gz * et.y;
+      }
+    }
   }
-  const double r0 = block_sum(f0, sh);
-  const double r1 = block_sum(f1, sh);
-  const double r2 = block_sum(f2, sh);
-  if (threadIdx.x == 0) {
-    double* o = part + ((size_t)a * gridDim.y + blockIdx.y) * 3;
-    o[0] = r0;
-    o[1] = r1;
-    o[2] = r2;
+#pragma unroll
+  for (int j = 0; j < PHASE_AG; ++j)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double v = f[j][k];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0) red[w][3 * j + k] = v;
+    }
+  __syncthreads();
+  if (threadIdx.x < 3 * gi.y) {
+    const int j = threadIdx.x / 3, k = threadIdx.x - 3 * j;
+    const double v = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+    part[((size_t)order[gi.x + j] * gridDim.y + blockIdx.y) * 3 + k] = v;
   }
 }
 
@@ -496,15 +526,15 @@ __device__ __forceinline__ void min_image(const PbcInfo& p, double d[3]) {
 }
 
 // Short-range ion-ion energy and forces with the minimum image above
-// (e_sr, total_energy.f90:133-239, nlp = 0).  One thread per atom i,
-// ordered sum over j; energy counted for j > i.
-__global__ void k_esr(const double* __restrict__ tau, int nat, const int* __restrict__ sp,
-                      const double* __restrict__ zv, PbcInfo pbc, double rckj,
-                      double* __restrict__ force, double* __restrict__ epart) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nat) return;
+// (e_sr, total_energy.f90:133-239, nlp = 0).  One wave per atom i: lane l
+// takes j = l, l + 64, ..., and the lanes' sums are combined by a fixed
+// butterfly; energy counted for j > i.
+__global__ void __launch_bounds__(64) k_esr(const double* __restrict__ tau, int nat, const int* __restrict__ sp,
+                                            const double* __restrict__ zv, PbcInfo pbc, double rckj,
+                                            double* __restrict__ force, double* __restrict__ epart) {
+  const int i = blockIdx.x;
   double e = 0, fx = 0, fy = 0, fz = 0;
-  for (int j = 0; j < nat; ++j) {
+  for (int j = threadIdx.x; j < nat; j += 64) {
     if (j == i) continue;
     double d[3] = {tau[3 * i] - tau[3 * j], tau[3 * i + 1] - tau[3 * j + 1], tau[3 * i + 2] - tau[3 * j + 2]};
     min_image(pbc, d);
@@ -521,10 +551,18 @@ __global__ void k_esr(const double* __restrict__ tau, int nat, const int* __rest
     fy += rep * d[1];
     fz += rep * d[2];
   }
-  force[3 * i] = fx;
-  force[3 * i + 1] = fy;
-  force[3 * i + 2] = fz;
-  epart[i] = e;
+  for (int o = 32; o > 0; o >>= 1) {
+    e += __shfl_xor(e, o, 64);
+    fx += __shfl_xor(fx, o, 64);
+    fy += __shfl_xor(fy, o, 64);
+    fz += __shfl_xor(fz, o, 64);
+  }
+  if (threadIdx.x == 0) {
+    force[3 * i] = fx;
+    force[3 * i + 1] = fy;
+    force[3 * i + 2] = fz;
+    epart[i] = e;
+  }
 }
 
 }  // namespace amdk

@@ -138,20 +138,57 @@ __device__ __forceinline__ double2 eigr_of(const double2* tab, int a, int W, int
   return zmul(zmul(t[m1 + hmax], t[W + m2 + hmax]), t[2 * W + m3 + hmax]);
 }
 
-// eigrxvps(G) = sum_a V_s(G) e^{-iGR_a};  eigrxrhos(G) = sum_a rho_s(G) e^{-iGR_a}
-__global__ void k_strfac(const int* __restrict__ mill, int ng, int nat, const int* __restrict__ sp,
-                         const double* __restrict__ vps, const double* __restrict__ rhos, int ldg,
-                         const double2* __restrict__ tab, int W, int hmax, double2* __restrict__ sv,
-                         double2* __restrict__ sr) {
-  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x) {
-    const int m1 = mill[3 * g], m2 = mill[3 * g + 1], m3 = mill[3 * g + 2];
-    double2 av = make_double2(0, 0), ar = make_double2(0, 0);
-    for (int a = 0; a < nat; ++a) {
-      const double2 e = eigr_of(tab, a, W, hmax, m1, m2, m3);
-      const double v = vps[(size_t)sp[a] * ldg + g], r = rhos[(size_t)sp[a] * ldg + g];
-      av.x += v * e.x; av.y += v * e.y;
-      ar.x += r * e.x; ar.y += r * e.y;
+// The structure-factor and local-force kernels walk the atoms in groups of
+// at most PHASE_AG atoms of one species (grp[q] = {first index into order[],
+// count, species}; order[] lists the atoms species by species), and stage the
+// group's x-axis phase rows e^{-i m1 s_1} (W = 2 hmax + 1 entries, grids up
+// to 192 per axis) in LDS, rows past the count as zeros; the loops over a
+// group run all PHASE_AG slots without branches, slots past the count taking
+// the group's last atom against a zero row.
+constexpr int PHASE_AG = 8, PHASE_WMAX = 193;
+
+__device__ __forceinline__ void stage_x_phases(const double2* __restrict__ tab, int W, const int* __restrict__ order,
+                                               int4 gi, double2 (*ex)[PHASE_WMAX]) {
+  for (int e = threadIdx.x; e < PHASE_AG * W; e += blockDim.x) {
+    const int j = e / W, m = e - j * W;
+    ex[j][m] = j < gi.y ? tab[(size_t)order[gi.x + j] * 3 * W + m] : make_double2(0.0, 0.0);
+  }
+}
+
+// eigrxvps(G) = sum_s V_s(G) sum_{a of s} e^{-iGR_a}, eigrxrhos(G) likewise
+// with rho_s (exp_igr.f90:4-140).  One G per thread; each atom group's sum
+// is scaled by V_s(G) and rho_s(G) of its species.  The y and z factors
+// come from the table (lanes of one plane-wave row read one entry).
+__global__ void __launch_bounds__(256) k_strfac(const int* __restrict__ mill, int ng, const int4* __restrict__ grp,
+                                                int ngrp, const int* __restrict__ order,
+                                                const double* __restrict__ vps, const double* __restrict__ rhos,
+                                                int ldg, const double2* __restrict__ tab, int W, int hmax,
+                                                double2* __restrict__ sv, double2* __restrict__ sr) {
+  __shared__ double2 ex[PHASE_AG][PHASE_WMAX];
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  const bool ok = g < ng;
+  const int gc = ok ? g : ng - 1;
+  const int m1 = mill[3 * gc] + hmax, m2 = mill[3 * gc + 1] + hmax, m3 = mill[3 * gc + 2] + hmax;
+  double2 av = make_double2(0.0, 0.0), ar = make_double2(0.0, 0.0);
+  for (int q = 0; q < ngrp; ++q) {
+    const int4 gi = grp[q];
+    __syncthreads();  // the previous group's rows are no longer read
+    stage_x_phases(tab, W, order, gi, ex);
+    __syncthreads();
+    double2 se = make_double2(0.0, 0.0);
+    for (int j = 0; j < gi.y; ++j) {
+      const double2* ty = tab + (size_t)order[gi.x + j] * 3 * W + W;
+      const double2 e = zmul(zmul(ex[j][m1], ty[m2]), ty[W + m3]);
+      se.x += e.x;
+      se.y += e.y;
     }
+    const double v = vps[(size_t)gi.z * ldg + gc], r = rhos[(size_t)gi.z * ldg + gc];
+    av.x += v * se.x;
+    av.y += v * se.y;
+    ar.x += r * se.x;
+    ar.y += r * se.y;
+  }
+  if (ok) {
     sv[g] = av;
     sr[g] = ar;
   }
