@@ -966,6 +966,260 @@ __global__ void __launch_bounds__(64 * NT) k_chol_inv(const double* __restrict__
   for (int e = tid; e < zr * n; e += 64 * NT) X[n + e % zr + (size_t)(e / zr) * ldx] = 0.0;
 }
 
+// k_chol_inv with look-ahead: the same elimination, row operations, tile
+// ownership and per-tile order of updates, with the panel elimination on
+// NT/4 waves of its own (thread 64 NT + x holds column x), so that it runs
+// while the tile waves apply the previous panel's MFMA updates.  Panel p's
+// pivot rows (raw), eliminated rows (R, RM) and 1/d (dinv_p) sit in LDS
+// buffers p & 1.  Tile wave J at panel p: waits until panel p is eliminated
+// (count `elim`), updates the tile row that holds the rows of panel p + 1
+// and publishes them (count `pub`), then updates its other tiles.  The
+// elimination waves wait until every tile wave has published panel p + 1,
+// then eliminate it.  A tile wave publishes panel p + 1 after its last read
+// of buffer (p + 1) & 1 (panel p - 1), so no buffer is overwritten while in
+// use.  Every wait polls at most CHOL_SPINS times; a non-positive pivot
+// (info as in k_chol_inv) or an exhausted wait (info = -1) sets `fail`, and
+// every wave then returns.
+constexpr int CHOL_SPINS = 1 << 22;
+
+template <int NT, int PB>
+__global__ void __launch_bounds__(80 * NT) k_chol_inv_la(const double* __restrict__ A, int lda, int n,
+                                                        double* __restrict__ X, int ldx, int nz,
+                                                        int* __restrict__ info, int info_base) {
+  constexpr int NM = 16 * NT, NMP = NM + 2, NE = NT / 4;
+  static_assert(PB % 4 == 0 && 16 % PB == 0, "panels of 4, 8 or 16 rows inside one tile row");
+  static_assert(NT % 4 == 0, "one elimination wave per 64 columns");
+  __shared__ double raw[2][PB][NM];
+  __shared__ double R[2][PB][NMP];
+  __shared__ double RM[2][PB][NMP];
+  __shared__ double dinv_p[2][PB];
+  __shared__ double dinv[NM];
+  __shared__ int pub, elim, fail;
+  const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  // earlier leaves ran before this launch on the same stream: the same value in every thread
+  if (info_base > 0 && *(volatile const int*)info != 0) return;
+  if (tid == 0) {
+    fail = 0;
+    pub = 0;
+    elim = 0;
+    if (info_base == 0) *info = 0;
+  }
+  for (int e = tid; e < 2 * PB * NMP; e += blockDim.x) {  // columns >= n stay zero
+    (&R[0][0][0])[e] = 0.0;
+    (&RM[0][0][0])[e] = 0.0;
+  }
+  // poll *cnt until it reaches target; false when the block has failed
+  auto wait_for = [&](int* cnt, int target) -> bool {
+    for (int s = 0;; ++s) {
+      if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(&fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)))
+        return false;
+      if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) >=
+          target)
+        return true;
+      if (s >= CHOL_SPINS) {
+        if (lane == 0) {
+          *info = -1;
+          __hip_atomic_store(&fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        return false;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  };
+  auto signal = [&](int* cnt) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane == 0) __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  };
+  const int np = (n + PB - 1) / PB;
+  // The two wave kinds run separate code; each passes the same four block
+  // barriers (set-up, first panel published, loops done, 1/sqrt(d) ready)
+  // unless the block fails, in which case every wave returns before the third.
+  if (wv >= NT) {
+    // elimination waves: column x; lane l < PB also carries the panel's own column k0 + l
+    const int x = 64 * (wv - NT) + lane;
+    __syncthreads();
+    __syncthreads();
+    for (int p = 0; p < np; ++p) {
+      const int k0 = p * PB, pn = min(PB, n - k0), b = p & 1, y = k0 + lane;
+      if (p > 0 && !wait_for(&pub, NT * p)) return;
+      double col[PB], pc[PB];
+#pragma unroll
+      for (int c = 0; c < PB; ++c) {
+        col[c] = (x < n && c < pn) ? raw[b][c][x] : 0.0;
+        pc[c] = (lane < pn && c < pn) ? raw[b][c][min(y, NM - 1)] : 0.0;
+      }
+      int bad = PB;  // first non-positive pivot of the panel
+      double dinv_q[PB];
+      // rows q >= pn (past n) are zero and run with pivot 1
+#pragma unroll
+      for (int q = 0; q < PB; ++q) {
+        const double dp = bcast_lane(pc[q], q);  // R_q(k_q), lane q
+        if (q < pn && !(dp > 0.0) && bad == PB) bad = q;
+        const double dq = q < pn ? dp : 1.0;
+        // 1 / d_q: hardware estimate and two Newton steps
+        double inv = __builtin_amdgcn_rcp(dq);
+        inv = fma(inv, fma(-dq, inv, 1.0), inv);
+        inv = fma(inv, fma(-dq, inv, 1.0), inv);
+        dinv_q[q] = inv;
+        const int kq = k0 + q;
+#pragma unroll
+        for (int c = q + 1; c < PB; ++c) {
+          const double f = bcast_lane(pc[q], c) * inv;  // R_q(k_c) / d_q, lane c
+          const int kc = k0 + c;
+          pc[c] -= f * ((y < kq || y >= kc) ? pc[q] : (y == kq ? 1.0 : 0.0));
+          col[c] -= f * col[q];  // exact for columns outside the panel
+        }
+      }
+      if (bad < PB) {  // the same in every elimination wave
+        if (lane == 0) {
+          if (wv == NT) *info = info_base + k0 + bad + 1;
+          __hip_atomic_store(&fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        return;
+      }
+      const bool in_panel = x >= k0 && x < k0 + PB;
+#pragma unroll
+      for (int c = 0; c < PB; ++c) {
+        if (c >= pn) continue;
+        if (x < n && !in_panel) {
+          R[b][c][x] = col[c];
+          RM[b][c][x] = x < k0 ? col[c] : 0.0;
+        }
+        if (wv == NT && lane < pn) {  // the first elimination wave writes the panel's columns
+          R[b][c][k0 + lane] = pc[c];
+          RM[b][c][k0 + lane] = lane < c ? pc[c] : (lane == c ? 1.0 : 0.0);
+        }
+        if (wv == NT && lane == 0) dinv_p[b][c] = dinv_q[c];
+      }
+      signal(&elim);
+    }
+    __syncthreads();
+    __syncthreads();
+  } else {
+    // tile waves: wave J holds tile column J
+    const int J = wv;
+    const int li = lane & 15, lk = lane >> 4, xj = 16 * J + li;  // this lane's column in its tiles
+    const bool jlive = 16 * J < n;
+    dbl4 acc[NT], accd;  // tile (I, J) for I = 0 .. NT-1; accd: lower copy of (J, J)
+#pragma unroll
+    for (int I = 0; I < NT; ++I)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = 16 * I + 4 * v + lk;
+        acc[I][v] = (I <= J && i <= xj && xj < n) ? A[i + (size_t)xj * lda] : 0.0;
+      }
+    accd = (dbl4){0.0, 0.0, 0.0, 0.0};
+    // raw[b][p][x] = row k0 + p of the combined array, from tile row k0 / 16
+    auto publish = [&](int k0, int b) {
+#pragma unroll
+      for (int I = 0; I < NT; ++I) {
+        if (I != k0 / 16) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = 16 * I + 4 * v + lk, p = i - k0;
+          if (p >= 0 && p < PB) raw[b][p][xj] = (I == J && xj < i) ? accd[v] : acc[I][v];
+        }
+      }
+    };
+    // the rank-PB update of tile row I by panel buffer b
+    auto upd_tile = [&](int I, int k0, int pn, int b, bool lower_live) {
+      // operand rows of F for tile row I: i = 16 I + li, panel step q = q0 + lk
+      const int i = 16 * I + li;
+      const bool below = i >= k0 + pn && i < n;
+#pragma unroll
+      for (int q0 = 0; q0 < PB; q0 += 4) {
+        const int q = q0 + lk;
+        const double a = (below && q < pn) ? -R[b][q][i] * dinv_p[b][q] : 0.0;
+#pragma unroll
+        for (int T = 0; T < NT; ++T) {
+          if (T != I) continue;
+          if (T < J) acc[T] = mfma16(a, R[b][q][xj], acc[T]);
+          else if (T > J) acc[T] = mfma16(a, RM[b][q][xj], acc[T]);
+          else {
+            acc[T] = mfma16(a, R[b][q][xj], acc[T]);
+            if (lower_live) accd = mfma16(a, RM[b][q][xj], accd);
+          }
+        }
+      }
+    };
+    // the panel's rows take their eliminated values
+    auto take_panel = [&](int I, int k0, int pn, int b) {
+#pragma unroll
+      for (int T = 0; T < NT; ++T) {
+        if (T != I) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = 16 * T + 4 * v + lk, pp = i - k0;
+          if (pp >= 0 && pp < pn && xj < n) {
+            acc[T][v] = R[b][pp][xj];
+            if (T == J) accd[v] = R[b][pp][xj];
+          }
+        }
+      }
+    };
+    __syncthreads();
+    if (jlive) publish(0, 0);
+    __syncthreads();
+    // fully unrolled over the panels, so that the tile rows touched by each
+    // panel (and the register tiles holding them) are known at compile time
+#pragma unroll
+    for (int p = 0; p < NT * 16 / PB; ++p) {
+      const int k0 = p * PB, pn = min(PB, n - k0), b = p & 1;
+      if (k0 >= n) break;
+      if (!wait_for(&elim, NE * (p + 1))) return;
+      const bool nxt = k0 + PB < n;
+      const int In = nxt ? (k0 + PB) / 16 : -1, Ip = k0 / 16;
+      const bool lower_live = 16 * J <= k0 + PB - 1;  // lower tiles right of the panel stay unchanged
+      // tile rows above or past the panel's reach, and below the diagonal
+      // while their lower part is not live, are left unchanged
+      auto live_row = [&](int I) { return !(16 * I + 15 < k0 + pn || 16 * I >= n) && !(I > J && !lower_live); };
+      if (jlive && nxt) {
+        if (live_row(In)) upd_tile(In, k0, pn, b, lower_live);
+        if (In == Ip) take_panel(Ip, k0, pn, b);
+        publish(k0 + PB, b ^ 1);
+      }
+      if (nxt) signal(&pub);
+      if (jlive) {
+#pragma unroll
+        for (int I = 0; I < NT; ++I) {
+          if (I == In) continue;
+          if (live_row(I)) upd_tile(I, k0, pn, b, lower_live);
+          if (I == Ip) take_panel(Ip, k0, pn, b);
+        }
+      }
+    }
+    __syncthreads();
+    if (jlive) {
+#pragma unroll
+      for (int I = 0; I < NT; ++I) {
+        if (I != J) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = 16 * I + 4 * v + lk;
+          if (xj == i && i < n) dinv[i] = 1.0 / sqrt(acc[I][v]);
+        }
+      }
+    }
+    __syncthreads();
+    if (jlive && xj < n) {
+#pragma unroll
+      for (int I = 0; I < NT; ++I)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int c = 16 * I + 4 * v + lk, r = xj;  // X(r, c) from element (c, r)
+          if (c >= n) continue;
+          double o;
+          if (r > c) o = 0.0;
+          else if (r == c) o = dinv[c];
+          else o = (I == J ? accd[v] : acc[I][v]) * dinv[c];
+          X[r + (size_t)c * ldx] = o;
+        }
+    }
+  }
+  const int zr = nz - n;
+  for (int e = tid; e < zr * n; e += blockDim.x) X[n + e % zr + (size_t)(e / zr) * ldx] = 0.0;
+}
+
 // W <- 1.5 I - 0.5 W (n x n, leading dimension ld): the factor T_k of the
 // Newton-Schulz iteration for S^-1/2.
 __global__ void k_ns_factor(double* __restrict__ W, int n, int ld) {
