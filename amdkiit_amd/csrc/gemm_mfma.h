@@ -16,6 +16,8 @@
 // relative to the tile sums whatever the total K.  Accumulation order is
 // fixed, so results are deterministic.
 #pragma once
+#include <utility>
+
 #include "fft_core.h"
 
 namespace amdk {
@@ -966,9 +968,16 @@ __global__ void __launch_bounds__(64 * NT) k_chol_inv(const double* __restrict__
   for (int e = tid; e < zr * n; e += 64 * NT) X[n + e % zr + (size_t)(e / zr) * ldx] = 0.0;
 }
 
+// f(integral_constant<int, 0>), f(..., 1), ... while f returns true
+template <class F, int... P>
+__device__ __forceinline__ void for_each_index(F& f, std::integer_sequence<int, P...>) {
+  (void)(f(std::integral_constant<int, P>{}) && ...);
+}
+
 // k_chol_inv with look-ahead: the same elimination, row operations, tile
 // ownership and per-tile order of updates, with the panel elimination on
-// NT/4 waves of its own (thread 64 NT + x holds column x), so that it runs
+// NT/4 waves of its own (waves NT/4 .. NT/2 - 1; lane l of the e-th holds
+// column 64 e + l), so that it runs
 // while the tile waves apply the previous panel's MFMA updates.  Panel p's
 // pivot rows (raw), eliminated rows (R, RM) and 1/d (dinv_p) sit in LDS
 // buffers p & 1.  Tile wave J at panel p: waits until panel p is eliminated
@@ -1034,9 +1043,12 @@ __global__ void __launch_bounds__(80 * NT) k_chol_inv_la(const double* __restric
   // The two wave kinds run separate code; each passes the same four block
   // barriers (set-up, first panel published, loops done, 1/sqrt(d) ready)
   // unless the block fails, in which case every wave returns before the third.
-  if (wv >= NT) {
+  // waves NE .. 2 NE - 1 eliminate (each on its own SIMD, beside fewer tile
+  // waves than SIMDs 0 and 1 carry), the others hold tile columns
+  const int ew = wv - NE;
+  if (ew >= 0 && ew < NE) {
     // elimination waves: column x; lane l < PB also carries the panel's own column k0 + l
-    const int x = 64 * (wv - NT) + lane;
+    const int x = 64 * ew + lane;
     __syncthreads();
     __syncthreads();
     for (int p = 0; p < np; ++p) {
@@ -1072,7 +1084,7 @@ __global__ void __launch_bounds__(80 * NT) k_chol_inv_la(const double* __restric
       }
       if (bad < PB) {  // the same in every elimination wave
         if (lane == 0) {
-          if (wv == NT) *info = info_base + k0 + bad + 1;
+          if (ew == 0) *info = info_base + k0 + bad + 1;
           __hip_atomic_store(&fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
         return;
@@ -1085,11 +1097,11 @@ __global__ void __launch_bounds__(80 * NT) k_chol_inv_la(const double* __restric
           R[b][c][x] = col[c];
           RM[b][c][x] = x < k0 ? col[c] : 0.0;
         }
-        if (wv == NT && lane < pn) {  // the first elimination wave writes the panel's columns
+        if (ew == 0 && lane < pn) {  // the first elimination wave writes the panel's columns
           R[b][c][k0 + lane] = pc[c];
           RM[b][c][k0 + lane] = lane < c ? pc[c] : (lane == c ? 1.0 : 0.0);
         }
-        if (wv == NT && lane == 0) dinv_p[b][c] = dinv_q[c];
+        if (ew == 0 && lane == 0) dinv_p[b][c] = dinv_q[c];
       }
       signal(&elim);
     }
@@ -1097,7 +1109,7 @@ __global__ void __launch_bounds__(80 * NT) k_chol_inv_la(const double* __restric
     __syncthreads();
   } else {
     // tile waves: wave J holds tile column J
-    const int J = wv;
+    const int J = wv < NE ? wv : wv - NE;
     const int li = lane & 15, lk = lane >> 4, xj = 16 * J + li;  // this lane's column in its tiles
     const bool jlive = 16 * J < n;
     dbl4 acc[NT], accd;  // tile (I, J) for I = 0 .. NT-1; accd: lower copy of (J, J)
@@ -1160,20 +1172,25 @@ __global__ void __launch_bounds__(80 * NT) k_chol_inv_la(const double* __restric
     __syncthreads();
     if (jlive) publish(0, 0);
     __syncthreads();
-    // fully unrolled over the panels, so that the tile rows touched by each
-    // panel (and the register tiles holding them) are known at compile time
-#pragma unroll
-    for (int p = 0; p < NT * 16 / PB; ++p) {
-      const int k0 = p * PB, pn = min(PB, n - k0), b = p & 1;
-      if (k0 >= n) break;
-      if (!wait_for(&elim, NE * (p + 1))) return;
+    // one copy of the panel step per panel index (compile-time p), so that
+    // the tile rows it touches, and the register tiles that hold them, are
+    // static; a step returns false to end the walk (past n, or failed)
+    bool failed = false;
+    auto step = [&](auto pc) -> bool {
+      constexpr int p = decltype(pc)::value;
+      constexpr int k0 = p * PB, b = p & 1, In = (k0 + PB) / 16, Ip = k0 / 16;
+      if (k0 >= n) return false;
+      const int pn = min(PB, n - k0);
+      if (!wait_for(&elim, NE * (p + 1))) {
+        failed = true;
+        return false;
+      }
       const bool nxt = k0 + PB < n;
-      const int In = nxt ? (k0 + PB) / 16 : -1, Ip = k0 / 16;
       const bool lower_live = 16 * J <= k0 + PB - 1;  // lower tiles right of the panel stay unchanged
       // tile rows above or past the panel's reach, and below the diagonal
       // while their lower part is not live, are left unchanged
       auto live_row = [&](int I) { return !(16 * I + 15 < k0 + pn || 16 * I >= n) && !(I > J && !lower_live); };
-      if (jlive && nxt) {
+      if (jlive && nxt && In < NT) {
         if (live_row(In)) upd_tile(In, k0, pn, b, lower_live);
         if (In == Ip) take_panel(Ip, k0, pn, b);
         publish(k0 + PB, b ^ 1);
@@ -1182,12 +1199,15 @@ __global__ void __launch_bounds__(80 * NT) k_chol_inv_la(const double* __restric
       if (jlive) {
 #pragma unroll
         for (int I = 0; I < NT; ++I) {
-          if (I == In) continue;
+          if (nxt && I == In) continue;
           if (live_row(I)) upd_tile(I, k0, pn, b, lower_live);
           if (I == Ip) take_panel(Ip, k0, pn, b);
         }
       }
-    }
+      return true;
+    };
+    for_each_index(step, std::make_integer_sequence<int, NT * 16 / PB>{});
+    if (failed) return;
     __syncthreads();
     if (jlive) {
 #pragma unroll
