@@ -228,49 +228,57 @@ __device__ __forceinline__ void xc_pz81(double rho, double& f, double& vr) {
 }
 
 __device__ __forceinline__ void xc_pbe(double rho, double sigma, double& f, double& vr, double& vs) {
+  // Constants in closed form (evaluated once, not per point):
+  //   CX = -(3/4) (3/pi)^(1/3), CS = 1 / (4 (9 pi^4)^(1/3)) (s^2 = CS sigma rho^-8/3),
+  //   CT = pi / (16 (3 pi^2)^(1/3)) (t^2 = CT sigma rho^-7/3), RS0 = (3/(4 pi))^(1/3).
+  // One cube root, one square root, two logarithms and one exponential per
+  // point; the powers of rho and the divisions share their reciprocals.
+  constexpr double CX = -0.7385587663820223, CS = 0.026121172985233605, CT = 0.0634682060977037;
+  constexpr double RS0 = 0.6203504908994001;
   // exchange
   const double kappa = 0.804, mu = 0.2195149727645171;
-  const double cx = -0.75 * cbrt(3.0 / M_PI);
+  const double irho = 1.0 / rho;
   const double r13 = cbrt(rho);
-  const double ex = cx * rho * r13;                           // rho * eps_x^LDA
-  const double cs = 1.0 / (4.0 * cbrt(9.0 * M_PI * M_PI * M_PI * M_PI));  // s^2 = cs sigma rho^{-8/3}
-  const double rm83 = 1.0 / (rho * rho * r13 * r13);
-  const double s2 = cs * sigma * rm83;
-  const double den = 1.0 + mu * s2 / kappa;
-  const double Fx = 1.0 + kappa - kappa / den;
-  const double dF = mu / (den * den);                         // dFx/ds2
+  const double ir13 = 1.0 / r13;
+  const double ex = CX * rho * r13;  // rho * eps_x^LDA
+  const double rm73 = irho * irho * ir13;
+  const double rm83 = rm73 * ir13;
+  const double s2 = CS * sigma * rm83;
+  const double iden = 1.0 / (1.0 + mu * s2 / kappa);
+  const double Fx = 1.0 + kappa - kappa * iden;
+  const double dF = mu * iden * iden;  // dFx/ds2
   const double fx = ex * Fx;
-  const double vrx = 4.0 / 3.0 * (ex / rho) * Fx - ex * dF * (8.0 / 3.0) * s2 / rho;
-  const double vsx = ex * dF * cs * rm83;
+  const double vrx = (4.0 / 3.0) * ex * irho * Fx - ex * dF * (8.0 / 3.0) * s2 * irho;
+  const double vsx = ex * dF * CS * rm83;
   // correlation: PW92 (PW_MOD constants) + H
   const double A = 0.0310907, a1 = 0.21370, B1 = 7.5957, B2 = 3.5876, B3 = 1.6382, B4 = 0.49294;
   const double beta = 0.06672455060314922, gamma = (1.0 - M_LN2) / (M_PI * M_PI);
-  const double rs = cbrt(3.0 / (4.0 * M_PI * rho));
+  const double rs = RS0 * ir13;
   const double srs = sqrt(rs);
   const double Q = 2.0 * A * (B1 * srs + B2 * rs + B3 * rs * srs + B4 * rs * rs);
   const double dQ = 2.0 * A * (0.5 * B1 / srs + B2 + 1.5 * B3 * srs + 2.0 * B4 * rs);
-  const double lq = log(1.0 + 1.0 / Q);
+  const double iQ = 1.0 / Q;
+  const double lq = log(1.0 + iQ);
   const double ec = -2.0 * A * (1.0 + a1 * rs) * lq;
-  const double dec = -2.0 * A * a1 * lq + 2.0 * A * (1.0 + a1 * rs) * dQ / (Q * Q + Q);
-  const double ct = M_PI / (16.0 * cbrt(3.0 * M_PI * M_PI));   // t^2 = ct sigma rho^{-7/3}
-  const double rm73 = 1.0 / (rho * rho * r13);
-  const double t2 = ct * sigma * rm73;
-  const double ee = exp(-ec / gamma);
+  const double dec = -2.0 * A * a1 * lq + 2.0 * A * (1.0 + a1 * rs) * dQ * iQ / (Q + 1.0);
+  const double t2 = CT * sigma * rm73;
+  const double ee = exp(-ec * (1.0 / gamma));
   const double Ac = (beta / gamma) / (ee - 1.0);
-  const double dAde = Ac * Ac * ee / beta;
+  const double dAde = Ac * Ac * ee * (1.0 / beta);
   const double y = Ac * t2;
-  const double N = 1.0 + y, Dn = 1.0 + y + y * y;
-  const double R = t2 * N / Dn;
-  const double arg = 1.0 + (beta / gamma) * R;
-  const double H = gamma * log(arg);
-  const double dRdt2 = N / Dn - y * y * (2.0 + y) / (Dn * Dn);
-  const double dRdA = -t2 * t2 * y * (2.0 + y) / (Dn * Dn);
-  const double dHdt2 = beta * dRdt2 / arg;
-  const double dHdA = beta * dRdA / arg;
-  const double drs = -rs / (3.0 * rho);
+  const double N = 1.0 + y;
+  const double iDn = 1.0 / (1.0 + y + y * y);
+  const double R = t2 * N * iDn;
+  const double iarg = 1.0 / (1.0 + (beta / gamma) * R);
+  const double H = -gamma * log(iarg);
+  const double dRdt2 = N * iDn - y * y * (2.0 + y) * iDn * iDn;
+  const double dRdA = -t2 * t2 * y * (2.0 + y) * iDn * iDn;
+  const double dHdt2 = beta * dRdt2 * iarg;
+  const double dHdA = beta * dRdA * iarg;
+  const double drs = -rs * irho * (1.0 / 3.0);
   const double fc = rho * (ec + H);
-  const double vrc = ec + H + rho * (dec * drs + dHdA * dAde * dec * drs + dHdt2 * (-7.0 / 3.0) * t2 / rho);
-  const double vsc = rho * dHdt2 * ct * rm73;
+  const double vrc = ec + H + rho * (dec * drs + dHdA * dAde * dec * drs) + dHdt2 * (-7.0 / 3.0) * t2;
+  const double vsc = rho * dHdt2 * CT * rm73;
   f = fx + fc;
   vr = vrx + vrc;
   vs = vsx + vsc;
