@@ -1,4 +1,6 @@
-"""CPU restatement of the one-workgroup Cholesky inverse (k_chol_inv): the
+"""CPU restatement of the one-workgroup Cholesky inverse (k_chol_inv, and
+k_chol_inv_la, which applies the same row operations to each tile in the same
+order, with the panel eliminations on waves of their own): the
 panel LDL^T elimination that carries L^-1 in the strictly lower triangle,
 with the kernel's tile ownership and skip rules (16 x 16 tiles, upper tiles
 hold S, lower tiles M = L^-1, the diagonal tile both; rank-PB updates of the
