@@ -1240,19 +1240,53 @@ __global__ void __launch_bounds__(80 * NT) k_chol_inv_la(const double* __restric
   for (int e = tid; e < zr * n; e += blockDim.x) X[n + e % zr + (size_t)(e / zr) * ldx] = 0.0;
 }
 
-// W <- 1.5 I - 0.5 W (n x n, leading dimension ld): the factor T_k of the
-// Newton-Schulz iteration for S^-1/2.
-__global__ void k_ns_factor(double* __restrict__ W, int n, int ld) {
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += gridDim.x * blockDim.x) {
-    const int i = e % n, j = e / n;
-    double* w = W + i + (size_t)j * ld;
-    *w = (i == j ? 1.5 : 0.0) - 0.5 * *w;
-  }
-}
 // W <- I (n x n)
 __global__ void k_identity(double* __restrict__ W, int n, int ld) {
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += gridDim.x * blockDim.x)
     W[e % n + (size_t)(e / n) * ld] = (e % n == e / n) ? 1.0 : 0.0;
+}
+
+// C = alpha op(A) op(B) + beta C (+ diag on the diagonal) for the small
+// band-space products (blocked Cholesky, Newton-Schulz, the projector
+// correction of the Lagrange matrix), FP64 on v_mfma_f64_16x16x4f64: one
+// block per 16 x 16 tile of C, its four waves each summing a quarter of K,
+// the quarters added in a fixed order.  op(X) = X or X^T (column-major
+// storage); beta == 0 ignores C's contents.
+__global__ void __launch_bounds__(256) k_gemm_small_mfma(int ta, int tb, int M, int N, int K, double alpha,
+                                                         const double* __restrict__ A, int lda,
+                                                         const double* __restrict__ B, int ldb, double beta,
+                                                         double* __restrict__ C, int ldc, double diag) {
+  __shared__ double red[3][4][64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
+  const int kq = ((K + 15) / 16) * 4;  // K per wave, a multiple of 4
+  const int kb = wave * kq, ke = min(K, kb + kq);
+  const int i = i0 + li, j = j0 + li;
+  dbl4 acc = (dbl4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int k0 = kb; k0 < ke; k0 += 4) {
+    const int k = k0 + lk;
+    const bool kin = k < ke;
+    const double a = (kin && i < M) ? (ta ? A[k + (size_t)i * lda] : A[i + (size_t)k * lda]) : 0.0;
+    const double b = (kin && j < N) ? (tb ? B[j + (size_t)k * ldb] : B[k + (size_t)j * ldb]) : 0.0;
+    acc = mfma16(a, b, acc);
+  }
+  if (wave > 0) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) red[wave - 1][v][lane] = acc[v];
+  }
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const double sum = ((acc[v] + red[0][v][lane]) + red[1][v][lane]) + red[2][v][lane];
+      const int r = i0 + 4 * v + lk, cidx = j0 + li;  // element (r, cidx) of this lane
+      if (r < M && cidx < N) {
+        double* cp = C + r + (size_t)cidx * ldc;
+        *cp = alpha * sum + (beta != 0.0 ? beta * *cp : 0.0) + (r == cidx ? diag : 0.0);
+      }
+    }
+  }
 }
 
 // C = alpha op(A) op(B) + beta C for the small band-block products of the
