@@ -298,14 +298,15 @@ static bool reg_ok(const amdk_ctx* c, int n) {
 // x tiles within one plane row: walking the columns across planes instead
 // measured slower, these passes being bound by memory), ZTX columns per
 // block for the z passes, which walk the n1 x n2 columns in one x-fastest
-// sequence, so no x tile is partly past the row end: 16, or
-// 32 where 16 * TM threads would leave the last wave partly empty and 32 * TM
-// fills whole waves (TM = 10: 320 threads, five full waves, instead of 160).
+// sequence, so no x tile is partly past the row end: 16 (at TM = 10 the
+// third wave of the 160 threads is half empty, yet 16 columns measured
+// faster than 32 or 64: smaller blocks leave fewer waves waiting at each of
+// the transform's barriers; 8 was slower).
 // HR half rows per block (x passes), PG band pairs sharing one V(r) tile in
 // the z pass of H|psi> (2, 4 and 8 measured equal within 1 %).
 template <typename T, int TM> struct RegTiles {
   static constexpr int TX = 16;
-  static constexpr int ZTX = (16 * TM) % 64 != 0 && (32 * TM) % 64 == 0 ? 32 : 16;
+  static constexpr int ZTX = 16;
   static constexpr int HR = 240 / (2 * TM);
   static constexpr int PG = sizeof(T) == 4 ? 4 : 2;
 };
