@@ -669,7 +669,7 @@ int amdk_ctx_create(const amdk_desc* d, amdk_ctx** out) {
     ctx->noalloc_scratch = noalloc_scratch(ctx->device);
     AMDK_TRY_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     AMDK_TRY_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
-    for (hipEvent_t* e : {&ctx->ev_fork, &ctx->ev_c0, &ctx->ev_nl})
+    for (hipEvent_t* e : {&ctx->ev_fork, &ctx->ev_nl})
       AMDK_TRY_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
     build_sphere(ctx, d->miller_pw, d->ngpw, ctx->spw);
     build_sphere(ctx, d->miller_rho, d->ngrho, ctx->srho);
@@ -731,7 +731,7 @@ void amdk_ctx_destroy(amdk_ctx* ctx) {
   for (void* p : ctx->owned) (void)hipFree(p);
   for (auto& s : ctx->pending) { (void)hipEventDestroy(s.a); (void)hipEventDestroy(s.b); }
   for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
-  for (hipEvent_t e : {ctx->ev_fork, ctx->ev_c0, ctx->ev_nl})
+  for (hipEvent_t e : {ctx->ev_fork, ctx->ev_nl})
     if (e) (void)hipEventDestroy(e);
   if (ctx->side) {
     (void)hipStreamSynchronize(ctx->side);

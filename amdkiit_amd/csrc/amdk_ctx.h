@@ -57,7 +57,7 @@ struct amdk_ctx {
   // second stream of the SCF step (work that only reads C0 runs beside the
   // density pass and V_KS) and the events that fork and join it
   hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_c0 = nullptr, ev_nl = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_nl = nullptr;
   amdk::Sphere spw, srho;
   amdk::FftPlan1D p1, p2, p3;
   // twiddles exp(-2 pi i t/n) in float2 and double2

@@ -19,6 +19,7 @@
 #include <utility>
 
 #include "fft_core.h"
+#include "wave.h"
 
 namespace amdk {
 
@@ -421,7 +422,7 @@ constexpr int UPM_MC = 32, UPM_NB = 128, UPM_K = 64;
 //   part[L]              = sum w_n DOTP(Y_n, Y_n)                 (CNORM)
 //   part[stride + L]     = max |Y_n(G)|                            (GEMAX)
 //   part[2 stride + L]   = sum w_n (finv_n vpp_g)^2 DOTP(Y_n, Y_n) (PCG ||P||^2)
-// for logical block L; k_fold_norms turns these into the step's reduction
+// for logical block L; k_fold_norms_pcg turns these into the step's reduction
 // slots in a fixed order.  part == nullptr: nothing is computed.
 struct UpdNorms {
   const double* vpp;
@@ -532,16 +533,25 @@ __device__ __forceinline__ void upd_store(const dbl4* acc, Row row, double* __re
 }
 
 // Fold the n block partials of upd_store into RED_BLOCKS partials per slot
-// (out + q * nred for q = 0 CNORM, 1 GEMAX (max), 2 ||P||^2), each a fixed
-// strided subset in a fixed order.
-__global__ void k_fold_norms(const double* __restrict__ part, int stride, int n, double* __restrict__ out, int nred) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= 3 * nred) return;
-  const int q = i / nred, r = i - q * nred;
-  const double* p = part + (size_t)q * stride;
-  double v = 0.0;
-  for (int j = r; j < n; j += nred) v = q == 1 ? fmax(v, p[j]) : v + p[j];
-  out[(size_t)q * nred + r] = v;
+// (out + q * RED_BLOCKS for q = 0 CNORM, 1 GEMAX (max), 2 ||P||^2), each a
+// fixed strided subset in a fixed order, in one block of 1024 threads; then,
+// in the same block, the PCG scalars of k_pcg_scalar from the folded ||P||^2
+// partials: one launch instead of two between the band update and the PCG
+// update.
+__global__ void __launch_bounds__(1024) k_fold_norms_pcg(const double* __restrict__ part, int stride, int n,
+                                                        double* __restrict__ out, double* st, int istep) {
+  __shared__ double sh[16];
+  double p2 = 0.0;
+  for (int i = threadIdx.x; i < 3 * RED_BLOCKS; i += blockDim.x) {
+    const int q = i / RED_BLOCKS, r = i - q * RED_BLOCKS;
+    const double* p = part + (size_t)q * stride;
+    double v = 0.0;
+    for (int j = r; j < n; j += RED_BLOCKS) v = q == 1 ? fmax(v, p[j]) : v + p[j];
+    out[i] = v;
+    if (q == 2) p2 += v;
+  }
+  p2 = block_sum(p2, sh);
+  if (threadIdx.x == 0) pcg_scalar_set(p2, st, istep);
 }
 
 // Block: 64 reals of k (4 waves x 16) x 128 columns n (8 MFMA tiles per

@@ -8,7 +8,7 @@
 namespace amdk {
 
 // ---------------------------------------------------------------------------
-// block reductions (256 threads)
+// block reductions (sh holds one double per wave of the block)
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   const int w = threadIdx.x / 64, l = threadIdx.x % 64;
@@ -249,21 +249,22 @@ __global__ void k_dbeta(const V* __restrict__ beta, const int* __restrict__ mill
 // k_pcg_scalar: st[0] = ||P||^2 of this step (the sum of its RED_BLOCKS
 // partials), st[2] = gamma = forget * st[0] / st[1] (0 at istep 1), then
 // st[1] = st[0].  istep is 1-based within one optimisation.
+__device__ __forceinline__ void pcg_scalar_set(double p2, double* st, int istep) {
+  st[0] = p2;
+  double gamma = 0.0;
+  if (istep > 1) {
+    const double forget = istep <= 10 ? 0.60 : (istep <= 20 ? 0.91 : 0.95);
+    gamma = forget * st[0] / st[1];
+  }
+  st[2] = gamma;
+  st[1] = st[0];
+}
 __global__ void k_pcg_scalar(const double* __restrict__ part, double* st, int istep) {
   __shared__ double sh[4];
   double v = 0.0;
   for (int i = threadIdx.x; i < RED_BLOCKS; i += blockDim.x) v += part[i];
   v = block_sum(v, sh);
-  if (threadIdx.x == 0) {
-    st[0] = v;
-    double gamma = 0.0;
-    if (istep > 1) {
-      const double forget = istep <= 10 ? 0.60 : (istep <= 20 ? 0.91 : 0.95);
-      gamma = forget * st[0] / st[1];
-    }
-    st[2] = gamma;
-    st[1] = st[0];
-  }
+  if (threadIdx.x == 0) pcg_scalar_set(v, st, istep);
 }
 
 // d = P + gamma d ; C0 += step d  (d stored in H; P = f^-1 K^-1 C2 formed on
