@@ -557,7 +557,7 @@ static void run_density(amdk_ctx* c, const void* C0, int nb, double* rho, bool n
     else launch_z_rho<T>(c, S, np, rho, p0 > 0, c->d_wpair + 2 * p0);
   }
   if (grouped) {
-    k_rho_combine<<<RED_BLOCKS, 256, 0, c->stream>>>(c->grid[1], nr, RHO_GROUPS, nr, rho, c->d_part + 8 * RED_BLOCKS);
+    k_rho_combine<RHO_GROUPS><<<RED_BLOCKS, 256, 0, c->stream>>>(c->grid[1], nr, nr, rho, c->d_part + 8 * RED_BLOCKS);
     AMDK_TRY_HIP(hipGetLastError());
   }
   if (norms_after) AMDK_TRY_HIP(hipStreamWaitEvent(c->stream, norms_after, 0));

@@ -331,13 +331,16 @@ __global__ void __launch_bounds__(256) k_xc(int kind, size_t nr, const double* _
   }
 }
 
-// rho = sum of G partial grids (fixed order) and the block sums of rho
-__global__ void __launch_bounds__(256) k_rho_combine(const double* __restrict__ part, size_t stride, int G, size_t n,
+// rho = sum of G partial grids (fixed order) and the block sums of rho; G is
+// a compile-time count so that a thread's G loads are all in flight at once
+template <int G>
+__global__ void __launch_bounds__(256) k_rho_combine(const double* __restrict__ part, size_t stride, size_t n,
                                                      double* __restrict__ rho, double* __restrict__ sums) {
   __shared__ double sh[4];
   double s = 0;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     double v = part[i];
+#pragma unroll
     for (int g = 1; g < G; ++g) v += part[g * stride + i];
     rho[i] = v;
     s += v;
@@ -373,6 +376,7 @@ __global__ void __launch_bounds__(256) k_c0_sums(const V* __restrict__ C, int ld
        e += (size_t)gridDim.x * blockDim.x) {
     const int g = (int)(e % npw), b0 = (int)(e / npw) * nbg, b1 = min(nb, b0 + nbg);
     const double g2 = g2pw[g];
+#pragma unroll 4
     for (int b = b0; b < b1; ++b) {
       const V c = C[(size_t)b * ldc + g];
       const double a2 = (double)c.x * c.x + (double)c.y * c.y;
