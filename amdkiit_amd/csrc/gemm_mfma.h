@@ -534,23 +534,27 @@ __device__ __forceinline__ void upd_store(const dbl4* acc, Row row, double* __re
 
 // Fold the n block partials of upd_store into RED_BLOCKS partials per slot
 // (out + q * RED_BLOCKS for q = 0 CNORM, 1 GEMAX (max), 2 ||P||^2), each a
-// fixed strided subset in a fixed order, in one block of 1024 threads; then,
-// in the same block, the PCG scalars of k_pcg_scalar from the folded ||P||^2
-// partials: one launch instead of two between the band update and the PCG
-// update.
-__global__ void __launch_bounds__(1024) k_fold_norms_pcg(const double* __restrict__ part, int stride, int n,
-                                                        double* __restrict__ out, double* st, int istep) {
-  __shared__ double sh[16];
-  double p2 = 0.0;
-  for (int i = threadIdx.x; i < 3 * RED_BLOCKS; i += blockDim.x) {
-    const int q = i / RED_BLOCKS, r = i - q * RED_BLOCKS;
-    const double* p = part + (size_t)q * stride;
-    double v = 0.0;
-    for (int j = r; j < n; j += RED_BLOCKS) v = q == 1 ? fmax(v, p[j]) : v + p[j];
-    out[i] = v;
-    if (q == 2) p2 += v;
+// fixed strided subset in a fixed order, in one block of RED_BLOCKS threads;
+// then, in the same block, the PCG scalars of k_pcg_scalar from the folded
+// ||P||^2 partials: one launch instead of two between the band update and
+// the PCG update.
+__global__ void __launch_bounds__(RED_BLOCKS) k_fold_norms_pcg(const double* __restrict__ part, int stride, int n,
+                                                              double* __restrict__ out, double* st, int istep) {
+  __shared__ double sh[RED_BLOCKS / 64];
+  const int r = threadIdx.x;  // one output per slot and thread (blockDim.x = RED_BLOCKS)
+  double v[3] = {0.0, 0.0, 0.0};
+  // slot by slot in the inner loop, so that a thread's loads of one j are in
+  // flight together
+  for (int j = r; j < n; j += RED_BLOCKS) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double x = part[(size_t)q * stride + j];
+      v[q] = q == 1 ? fmax(v[q], x) : v[q] + x;
+    }
   }
-  p2 = block_sum(p2, sh);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) out[(size_t)q * RED_BLOCKS + r] = v[q];
+  const double p2 = block_sum(v[2], sh);
   if (threadIdx.x == 0) pcg_scalar_set(p2, st, istep);
 }
 
