@@ -19,10 +19,10 @@ namespace amdk {
 constexpr int OVR_G = 16;
 template <typename V>
 __global__ void __launch_bounds__(64 * OVR_G) k_ovl_reduce(const double* __restrict__ part, int nsplit, int M, int N,
-                                                    const V* __restrict__ X, int ldx, const V* __restrict__ Y, int ldy,
-                                                    double alpha, double* __restrict__ A, int lda, int g0,
-                                                    double beta, const double* __restrict__ occ, int sym,
-                                                    int tile) {
+                                                            const V* __restrict__ X, int ldx, const V* __restrict__ Y,
+                                                            int ldy, double alpha, double* __restrict__ A, int lda,
+                                                            int g0, double beta, const double* __restrict__ occ,
+                                                            int sym, int tile) {
   __shared__ double sh[OVR_G][64];
   const int l = threadIdx.x & 63, grp = threadIdx.x >> 6;
   const size_t MN = (size_t)M * N;
